@@ -1,0 +1,206 @@
+"""Benchmark of the no-three-in-line hot path on one MI355X.
+
+Headline (value): n=75 population scoring -- P = 65,536 seeded states,
+exact collinear-triple count plus the per-slot badness vector int32[150] of
+every state -- in states scored per second, inputs resident in HBM.  One
+step is one scoring pass over the whole population.
+
+Also reported: the same population's kernel roofline (INT32 VALU), a CPU
+baseline (the oracle's OpenMP restatement of the reference loop, timed on the
+host's cores on a bounded sample), and the replica-exchange tempering rate
+(64 ladders x 64 temperatures at n=75, proposals per second).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+N = 75
+POPULATION = 65536
+SEED = 0
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def slope_ops_per_state(n: int) -> int:
+    """Algorithmic work of the direction-histogram scorer: for every anchor a
+    and point b (m^2 ordered pairs, a == b included as the zero bin): key
+    difference, table lookup, histogram increment, count read-back,
+    accumulate -- 5 integer operations per pair."""
+    m = 2 * n
+    return 5 * m * m
+
+
+def determinant_ops_per_state(n: int) -> int:
+    """6 C(m,3) + 2 C(m,2): the reference loop's integer work (accelerated.py:27-36)."""
+    m = 2 * n
+    return 6 * (m * (m - 1) * (m - 2) // 6) + 2 * (m * (m - 1) // 2)
+
+
+def traffic_from_profile() -> float | None:
+    """HBM bytes per scoring launch from the committed rocprofv3 PMC summary."""
+    path = ROOT / "profiles" / "score_slope_n75_P65536_pmc.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(seconds: float = 12.0) -> dict:
+    import numpy as np
+
+    import __graft_entry__ as entry
+
+    oracle = entry.load_oracle()
+    cores = oracle.cpu_count()
+    p, q = oracle.generate_states(SEED, N, 4096)
+    oracle.score_population(p[:64], q[:64], threads=cores)  # warm up the pool
+    t0 = time.perf_counter()
+    oracle.score_population(p[:1024], q[:1024], threads=cores)
+    probe = time.perf_counter() - t0
+    reps = max(1, int(seconds / max(probe * 4, 1e-6)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.score_population(p, q, threads=cores)
+    elapsed = time.perf_counter() - t0
+    states = reps * len(p)
+    t0 = time.perf_counter()
+    oracle.score_population(p[:128], q[:128], threads=1)
+    single = 128 / (time.perf_counter() - t0)
+    return {"value": states / elapsed, "unit": "states/s", "cores": cores, "kind": "port",
+            "sample": f"{states} n=75 seeded states (the first 4096 of the population, {reps} passes), "
+                      f"oracle C restatement of accelerated.score_and_badness, OpenMP over states",
+            "single_thread_states_per_s": single}
+
+
+def tempering_rate(pkg, torch, sweeps: int = 100) -> dict:
+    """64 ladders x 64 temperatures (geometric 0.05..2.0) at n=75, 150
+    proposals per sweep, exchange every sweep, graph-captured."""
+    import numpy as np
+
+    replicas, ladder, steps = 4096, 64, 150
+    dev = pkg._lib.device(0)
+    p, q = pkg.generate_states(SEED + 1, N, replicas, device=dev)
+    cols = torch.stack((p, q), dim=2).reshape(replicas, 2 * N).cpu().numpy()
+    temps = np.tile(np.geomspace(0.05, 2.0, ladder), replicas // ladder).astype(np.float32)
+    chains = pkg.TemperingChains(cols, temps, seed=3)
+    chains.run(5, steps, ladder)
+    torch.cuda.synchronize()
+    before = chains.proposals.cpu().numpy().copy()
+    t0 = time.perf_counter()
+    chains.run(sweeps, steps, ladder)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    after = chains.proposals.cpu().numpy()
+    chains.validate(range(0, replicas, 97))
+    proposals = sweeps * steps * replicas
+    return {"replicas": replicas, "ladders": replicas // ladder, "sweeps": sweeps, "steps_per_sweep": steps,
+            "seconds": elapsed, "proposals_per_s": proposals / elapsed,
+            "valid_proposals_per_s": float(after[0] - before[0]) / elapsed,
+            "accepted_per_s": float(after[1] - before[1]) / elapsed,
+            "best_E": int(chains.best_scores.min().item())}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-temper", action="store_true", help="skip the tempering rate")
+    ap.add_argument("--method", type=int, default=0, help="0 auto (direction histograms), 1 determinant")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine shards by whole "
+              f"populations/ladders and runs no collectives", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import __graft_entry__ as entry
+
+    pkg = entry.load_package()
+    dev = pkg._lib.device(0)
+    log(f"bench: device {torch.cuda.get_device_name(dev)}; generating n={N} P={POPULATION}")
+    p, q = pkg.generate_states(SEED, N, POPULATION, device=dev)
+    scores = torch.empty(POPULATION, dtype=torch.int32, device=dev)
+    bad = torch.empty((POPULATION, 2 * N), dtype=torch.int32, device=dev)
+    for _ in range(args.warmup):
+        pkg.score_population_device(p, q, scores, bad, method=args.method, check=False)
+    pkg.check_status(dev)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        pkg.score_population_device(p, q, scores, bad, method=args.method, check=False)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pkg.check_status(dev)
+    kernel_ms = start.elapsed_time(end) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    value = POPULATION * args.steps / wall
+    log(f"bench: {args.steps} steps, {ms_per_step:.3f} ms/step wall, {kernel_ms:.3f} ms/launch (events)")
+
+    # parity spot check of this run's outputs on a subsample
+    import numpy as np
+
+    oracle = entry.load_oracle()
+    idx = np.arange(0, POPULATION, POPULATION // 64)
+    hp, hq = p[idx].cpu().numpy(), q[idx].cpu().numpy()
+    ref_s, ref_b = oracle.score_population(hp, hq)
+    exact = bool(np.array_equal(scores[idx].cpu().numpy(), ref_s) and np.array_equal(bad[idx].cpu().numpy(), ref_b))
+    if not exact:
+        log("bench: PARITY FAILURE on the subsample")
+
+    ops = slope_ops_per_state(N) if args.method != 1 else determinant_ops_per_state(N)
+    achieved = POPULATION * ops / (kernel_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "TOP/s",
+                "frac": achieved / PEAK_VALU_TOPS, "traffic": traffic_from_profile(),
+                "ops_per_state": ops, "kernel_ms": kernel_ms,
+                "triple_tests_per_s": POPULATION * (2 * N) * (2 * N - 1) * (2 * N - 2) / 6 / (kernel_ms * 1e-3)}
+
+    extra = {}
+    if not args.no_temper:
+        log("bench: tempering rate")
+        extra["tempering_n75_64x64"] = tempering_rate(pkg, torch)
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        cpu = cpu_baseline()
+        cpu["gpu_over_cpu"] = value / cpu["value"]
+
+    line = {
+        "metric": "n=75 states scored/s + time-to-E≤12 (learned vs uniform proposals), 1 GPU",
+        "value": value, "unit": "states/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "n=75 population scoring: exact collinear-triple count + badness int32[150] per state",
+                   "n": N, "population": POPULATION, "seed": SEED,
+                   "method": "direction-histogram" if args.method != 1 else "determinant"},
+        "parity_subsample_exact": exact,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if exact else 1
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

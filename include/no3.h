@@ -1,0 +1,161 @@
+/*
+ * no3 — C ABI of the gfx950 (MI355X) engine for the no-three-in-line hot
+ * path: exact collinear-triple scoring of P/Q permutation states, 2-switch
+ * move deltas, and replica-exchange tempering.
+ *
+ * Conventions (all entry points):
+ *   - returns 0 on success, a negative status otherwise (N3_E*); nothing
+ *     throws across the ABI; n3_last_error() describes the last failure;
+ *   - array arguments are DEVICE pointers owned by the caller unless the name
+ *     says host_; `stream` is a hipStream_t (NULL = the null stream); calls
+ *     enqueue work and return without synchronising unless stated;
+ *   - a state of size n is two int32 permutations P, Q of 0..n-1 stored as
+ *     rows of planar arrays p[count][n], q[count][n]; slot s of the state is
+ *     the point (s >> 1, s & 1 ? Q[s >> 1] : P[s >> 1]) and m = 2n;
+ *   - columns outside [0, n) are rejected with N3_EINVAL; nothing else about
+ *     the states is assumed (coincident points P[r] == Q[r] are counted
+ *     exactly as the reference's determinant loops count them).
+ */
+#ifndef NO3_H
+#define NO3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define N3_OK 0
+#define N3_EINVAL (-22)   /* bad argument (shape, range, n) */
+#define N3_ENODEV (-19)   /* no usable gfx950 device */
+#define N3_ENOMEM (-12)
+#define N3_EHIP (-1000)   /* a HIP call failed; -1000 - hipError_t */
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int n3_version(void);
+
+/* Text of the last error recorded by this thread (never NULL). */
+const char* n3_last_error(void);
+
+/* Checks that `device` is a gfx950 GPU and makes it current.  No reference
+ * counterpart beyond gpu.py:50-51 ("no CUDA device is available"). */
+int n3_device_init(int device);
+
+/* Direction table of size n (see csrc/no3_common.h): fills `host_table`
+ * (uint16[(2n-1)^2]) when not NULL and reports the number of primitive
+ * directions.  Host-only.  Basis: verifier_b.py:8-19. */
+int n3_dirtable(int n, uint16_t* host_table, int* directions, int* hist_words);
+
+/* Seeded population of `count` states, state i being the generator's state
+ * (first_index + i) for `seed` (Philox4x32-10 Fisher-Yates shuffles of P and
+ * Q, each collision P[r] == Q[r] repaired by swapping Q[r] with Q of a random
+ * other row).  Replaces the host loop of cli.py:203-205 / state.py:88-95. */
+int n3_generate_states(uint64_t seed, int n, int64_t count, int64_t first_index, int32_t* p, int32_t* q,
+                       void* stream);
+
+#define N3_SCORE_AUTO 0          /* direction histograms, determinant for coincident points */
+#define N3_SCORE_DETERMINANT 1   /* i<j<k determinant loop only */
+#define N3_SCORE_SLOPE 2         /* direction histograms (as AUTO) */
+
+/* Exact collinear-triple count of every state, and optionally the number of
+ * collinear triples through every slot (int32 badness[count][2n]).
+ * Replaces the kernels score_population (gpu.py:13-42, called by
+ * score_population_gpu gpu.py:69-77) and score_population_badness
+ * (gpu_device_search.py:118-161); CPU equivalent accelerated.py:14-37. */
+int n3_score_population(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores,
+                        int32_t* badness_or_null, int method, void* stream);
+
+/* Synchronises `stream` and reports whether the scoring launches since the
+ * last call met a column outside [0, n) (N3_EINVAL; the affected scores are
+ * -1).  The reference has no range check (gpu.py:61-66 checks shapes only). */
+int n3_score_status(void* stream);
+
+/* Score after every 2-switch of slot u with slot v of ONE state, int32 with
+ * the sentinel 1<<30 for illegal v.  Replaces two_switch_scores
+ * (gpu_delta.py:18-75; host wrapper score_two_switches_gpu gpu_delta.py:86-116). */
+int n3_two_switch_scores(const int32_t* p, const int32_t* q, const int32_t* badness, int n, int u,
+                         int current_score, int32_t* scores, void* stream);
+
+/* accelerated.evaluate_two_switches (accelerated.py:122-195) on the device:
+ * int64 scores[2n] with the sentinel 1<<60; badness is int64[2n]. */
+int n3_evaluate_two_switches(const int32_t* p, const int32_t* q, const int64_t* badness, int n, int u,
+                             int64_t current_score, int64_t* scores, void* stream);
+
+/* accelerated.evaluate_transpositions (accelerated.py:40-119) on the device:
+ * int64 scores[n] with the sentinel 1<<60; badness is int64[2n]. */
+int n3_evaluate_transpositions(const int32_t* p, const int32_t* q, const int64_t* badness, int n, int layer,
+                               int row, int64_t current_score, int64_t* scores, void* stream);
+
+/* Full 2-switch table of a population: table[s][u][v] is
+ * evaluate_two_switches(state s, u, scores[s], badness[s])[v] as int32 with
+ * the sentinel 1<<30 (the all-anchor sweep of analysis.py:57-74).  `scores`
+ * and `badness` (int32) are the states' current values. */
+int n3_delta_table(const int32_t* p, const int32_t* q, const int32_t* scores, const int32_t* badness, int n,
+                   int64_t count, int32_t* table, void* stream);
+
+/* ---- replica-exchange tempering (tempering.py:19-225) ------------------ */
+
+/* Device state of R replicas of size n, slot-ordered like the reference's
+ * TemperingChains arrays (tempering.py:138-152).  All pointers are device
+ * pointers owned by the caller. */
+typedef struct n3_chains {
+    int n;
+    int replicas;
+    int32_t* columns;        /* [R][2n] interleaved P0,Q0,P1,Q1,... */
+    int32_t* badness;        /* [R][2n] */
+    int32_t* scores;         /* [R] */
+    int32_t* best_columns;   /* [R][2n] */
+    int32_t* best_scores;    /* [R] */
+    int32_t* accepted;       /* [R] */
+    const int32_t* reference;/* [2n] umbrella reference state */
+    int32_t* removed_points; /* [R] */
+    const uint32_t* move_thresholds;  /* [R][k_move]: floor(2^32 exp(-d/T_r)) */
+    int k_move;
+    const uint32_t* exchange_thresholds; /* [R][k_exchange] for pair (r, r+1) */
+    const int8_t* exchange_sign;         /* [R] sign of 1/T_r - 1/T_{r+1} */
+    int k_exchange;
+    uint32_t* counters;      /* [4]: sweep, round, exchanges (low), exchanges (high) */
+    int64_t* proposals;      /* [2]: valid proposals (total), accepted moves (total) */
+} n3_chains;
+
+/* `steps` proposals per replica (one sweep), TemperingChains.advance
+ * (tempering.py:158-167, kernel walk tempering.py:29-115), with the umbrella
+ * strength 0.  Proposal u, v and the acceptance word come from Philox keyed
+ * by `seed` with counter (step, sweep, slot, "MOVE"); `sweep` is read from
+ * counters[0], which the call then increments. */
+int n3_temper_advance(const n3_chains* chains, int steps, int bad_bias, uint64_t seed, void* stream);
+
+/* Adjacent-temperature exchange inside ladders of `ladder` slots
+ * (TemperingChains.exchange, tempering.py:169-191) on the device; pairs start
+ * at round % 2 where round = counters[1], which the call then increments.
+ * Accepted pairs swap columns, badness, scores and removed_points. */
+int n3_temper_exchange(const n3_chains* chains, int ladder, uint64_t seed, void* stream);
+
+/* `sweeps` x (advance(steps); exchange(ladder) if ladder > 1) replayed from a
+ * captured hipGraph (the epoch loop of tempering.py:275-278).  The graph is
+ * cached per (chains, steps, ladder, bad_bias, seed, stream) and rebuilt when
+ * they change. */
+int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int ladder, int bad_bias, uint64_t seed,
+                  void* stream);
+
+/* Drops cached graphs (all of them). */
+void n3_temper_release(void);
+
+/* Host tables of the integer acceptance tests (no device work).
+ * Metropolis: host_out[r][d] = floor(2^32 exp(-d / T_r)) for 1 <= d < k
+ * (clamped to 2^32-1); a proposal with integer energy change d > 0 is
+ * accepted iff its Philox word < host_out[r][min(d, k-1)] (the law of
+ * log(U) < -d/T in tempering.py:92-95).
+ * Exchange: for the pair (a, a+1) of a ladder, c = 1/T_a - 1/T_{a+1},
+ * host_sign[a] = sign(c), host_out[a][d] = floor(2^32 exp(-|c| d)); the pair
+ * swaps iff c (E_a - E_{a+1}) >= 0 or the word < host_out[a][min(|E_a -
+ * E_{a+1}|, k-1)] (tempering.py:181-182). */
+int n3_metropolis_thresholds(const float* host_temperatures, int replicas, int k, uint32_t* host_out);
+int n3_exchange_thresholds(const float* host_temperatures, int replicas, int ladder, int k, uint32_t* host_out,
+                           int8_t* host_sign);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NO3_H */

@@ -1,0 +1,33 @@
+"""gfx950 (MI355X) engine for the no-three-in-line hot path.
+
+Exact collinear-triple scoring of P/Q permutation populations, 2-switch move
+scores and replica-exchange tempering, computed by hand-written HIP kernels
+in libno3.so (C ABI: include/no3.h).  The module names mirror the reference
+package's (gpu, gpu_delta, tempering, state, io), so callers can switch by
+changing the import.
+"""
+from . import _lib
+from ._lib import SCORE_AUTO, SCORE_DETERMINANT, SCORE_SLOPE, SENTINEL32, SENTINEL64
+from .gpu import (
+    check_status,
+    generate_states,
+    score_population_badness,
+    score_population_device,
+    score_population_gpu,
+)
+from .gpu_delta import (
+    delta_table,
+    delta_table_device,
+    evaluate_transpositions,
+    evaluate_two_switches,
+    score_two_switches_gpu,
+)
+from .state import PairPermutations, points_to_permutations, random_state
+from .tempering import TemperingChains
+
+__all__ = [
+    "SCORE_AUTO", "SCORE_DETERMINANT", "SCORE_SLOPE", "SENTINEL32", "SENTINEL64",
+    "check_status", "generate_states", "score_population_badness", "score_population_device", "score_population_gpu",
+    "delta_table", "delta_table_device", "evaluate_transpositions", "evaluate_two_switches", "score_two_switches_gpu",
+    "PairPermutations", "points_to_permutations", "random_state", "TemperingChains", "_lib",
+]

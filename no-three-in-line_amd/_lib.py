@@ -1,0 +1,131 @@
+"""ctypes binding of libno3.so (the C ABI declared in include/no3.h).
+
+torch is imported first on purpose: it loads the HIP runtime
+(libamdhip64.so.7) that libno3.so then shares, so device pointers and streams
+from torch are valid in the library.  There is no CPU fallback: every entry
+point raises if the library or a gfx950 device is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  (shares its HIP runtime with libno3.so)
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libno3.so"
+
+N3_OK = 0
+N3_EINVAL = -22
+N3_ENODEV = -19
+N3_ENOMEM = -12
+N3_EHIP = -1000
+
+SCORE_AUTO = 0
+SCORE_DETERMINANT = 1
+SCORE_SLOPE = 2
+
+SENTINEL32 = 1 << 30
+SENTINEL64 = 1 << 60
+
+
+class Chains(ctypes.Structure):
+    """n3_chains (include/no3.h)."""
+
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("replicas", ctypes.c_int),
+        ("columns", ctypes.c_void_p),
+        ("badness", ctypes.c_void_p),
+        ("scores", ctypes.c_void_p),
+        ("best_columns", ctypes.c_void_p),
+        ("best_scores", ctypes.c_void_p),
+        ("accepted", ctypes.c_void_p),
+        ("reference", ctypes.c_void_p),
+        ("removed_points", ctypes.c_void_p),
+        ("move_thresholds", ctypes.c_void_p),
+        ("k_move", ctypes.c_int),
+        ("exchange_thresholds", ctypes.c_void_p),
+        ("exchange_sign", ctypes.c_void_p),
+        ("k_exchange", ctypes.c_int),
+        ("counters", ctypes.c_void_p),
+        ("proposals", ctypes.c_void_p),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+
+# name -> (restype, argtypes); every symbol of include/no3.h
+SIGNATURES = {
+    "n3_version": (_i, []),
+    "n3_last_error": (ctypes.c_char_p, []),
+    "n3_device_init": (_i, [_i]),
+    "n3_dirtable": (_i, [_i, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "n3_generate_states": (_i, [_u64, _i, _i64, _i64, _vp, _vp, _vp]),
+    "n3_score_population": (_i, [_vp, _vp, _i, _i64, _vp, _vp, _i, _vp]),
+    "n3_score_status": (_i, [_vp]),
+    "n3_two_switch_scores": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
+    "n3_evaluate_two_switches": (_i, [_vp, _vp, _vp, _i, _i, _i64, _vp, _vp]),
+    "n3_evaluate_transpositions": (_i, [_vp, _vp, _vp, _i, _i, _i, _i64, _vp, _vp]),
+    "n3_delta_table": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp]),
+    "n3_temper_advance": (_i, [ctypes.POINTER(Chains), _i, _i, _u64, _vp]),
+    "n3_temper_exchange": (_i, [ctypes.POINTER(Chains), _i, _u64, _vp]),
+    "n3_temper_run": (_i, [ctypes.POINTER(Chains), _i, _i, _i, _i, _u64, _vp]),
+    "n3_temper_release": (None, []),
+    "n3_metropolis_thresholds": (_i, [_vp, _i, _i, _vp]),
+    "n3_exchange_thresholds": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+_initialised_devices: set[int] = set()
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises RuntimeError if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+            L = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    """Maps an n3 status to the reference's exception types."""
+    if status == N3_OK:
+        return
+    message = lib().n3_last_error().decode(errors="replace")
+    if status == N3_EINVAL:
+        raise ValueError(message)
+    raise RuntimeError(f"no3 status {status}: {message}")
+
+
+def device(index: int | None = None) -> torch.device:
+    """The current gfx950 device; RuntimeError if there is none (gpu.py:50-51)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is available")
+    idx = torch.cuda.current_device() if index is None else int(index)
+    if idx not in _initialised_devices:
+        with torch.cuda.device(idx):
+            check(lib().n3_device_init(idx))
+        _initialised_devices.add(idx)
+    return torch.device("cuda", idx)
+
+
+def stream_handle(dev: torch.device) -> int:
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()

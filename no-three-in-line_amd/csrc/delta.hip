@@ -1,0 +1,277 @@
+// 2-switch move scores: the reference's incremental formula
+//
+//   score(u, v) = current - (badness[u] + badness[v] - joint_old) + new_affected
+//
+// (accelerated.py:153-194, gpu_delta.py:42-74) for the switch of slots u, v
+// to (row_u, col_v), (row_v, col_u), with the same legality rules and
+// sentinels.  joint_old counts the points o != u, v on the line (u, v) (or
+// coincident with u); new_affected counts the collinear triples of the moved
+// state through u' or v', which is bad'(u') + bad'(v') - joint'(u', v'), each
+// bad' the sum of C(cnt, 2) over the direction histogram seen from u' (v').
+// One wavefront evaluates one (state, u, v) at a time in O(m) table lookups
+// instead of the reference's O(m^2) determinant pair loops.
+#include "runtime.h"
+#include "wave_ops.h"
+
+namespace no3 {
+
+struct SwitchArgs {
+  const int32_t* p;
+  const int32_t* q;
+  int n;
+  long long states;
+  int u_fixed;               // >= 0: this anchor for every state; -1: every anchor (table)
+  int layer;                 // -1: every partner slot; 0 / 1: partners of that layer only
+  int splits;                // wavefronts per (state, anchor)
+  const void* badness;       // [states][m], int32 or int64
+  const void* scores;        // [states], int32 or int64; NULL: use `current`
+  long long current;
+  void* out;                 // [states][anchors][width]
+  int width;                 // m (partner slot v) or n (partner row)
+  int* status;
+};
+
+template <typename T>
+__device__ inline long long load_value(const void* base, long long index) {
+  return (long long)reinterpret_cast<const T*>(base)[index];
+}
+
+template <int CH, typename BT, typename OT>
+__global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const uint16_t* __restrict__ table_g,
+                                                            DirTableInfo info) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  stage_table(table_g, table, info.entries);
+  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+
+  const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
+  const int zero_id = info.directions;
+  const long long anchors = a.u_fixed >= 0 ? 1 : m;
+  const long long items = a.states * anchors * a.splits;
+  for (long long w = (long long)blockIdx.x * waves + wave; w < items; w += (long long)gridDim.x * waves) {
+    const int split = (int)(w % a.splits);
+    const long long item = w / a.splits;
+    const long long s = item / anchors;
+    const int u = a.u_fixed >= 0 ? a.u_fixed : (int)(item % anchors);
+    OT* out = reinterpret_cast<OT*>(a.out) + item * a.width;
+
+    int key[CH];
+    int outside = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      key[c] = 0;
+      if (k < m) {
+        const int col = (k & 1) ? a.q[s * n + (k >> 1)] : a.p[s * n + (k >> 1)];
+        outside |= (col < 0 || col >= n);
+        key[c] = (k >> 1) * width + col;
+      }
+    }
+    if (__any(outside)) {
+      if (lane == 0) atomicOr(a.status, 1);
+      continue;
+    }
+    // lanes past the last slot hold slot 0's key (always a valid point)
+    const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (lane + kWave * c >= m) key[c] = key0;
+    const int trash = trash_word(info);
+    const int ru = u >> 1;
+    const int key_u = slot_value<CH>(key, u);
+    const int cu = key_u - ru * width;
+    const int cu_other = slot_value<CH>(key, u ^ 1) - ru * width;
+    // direction from u to every slot (joint_old)
+    int idu[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) idu[c] = table[key[c] - key_u + offset];
+    const long long bad_u = load_value<BT>(a.badness, s * m + u);
+    const long long current = a.scores ? load_value<BT>(a.scores, s) : a.current;
+
+    const int partners = a.layer >= 0 ? n : m;
+    for (int t = split; t < partners; t += a.splits) {
+      const int v = a.layer >= 0 ? 2 * t + a.layer : t;
+      const int rv = v >> 1;
+      const int key_v = slot_value<CH>(key, v);
+      const int cv = key_v - rv * width;
+      const int cv_other = slot_value<CH>(key, v ^ 1) - rv * width;
+      // accelerated.py:146-151, gpu_delta.py:35-40
+      if (v == u || rv == ru || cv == cu_other || cu == cv_other) {
+        if (lane == 0) out[t] = (OT)(sizeof(OT) == 8 ? kSentinel64 : (long long)kSentinel32);
+        continue;
+      }
+      const int id_uv = table[key_v - key_u + offset];
+      int joint_old = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        const bool on = (k < m) && k != u && k != v && (idu[c] == id_uv || idu[c] == zero_id);
+        joint_old += __popcll(__ballot(on));
+      }
+      const int key_u2 = ru * width + cv;  // u' = (row_u, col_v)
+      const int key_v2 = rv * width + cu;  // v' = (row_v, col_u)
+      const int id_u2v2 = table[key_v2 - key_u2 + offset];
+      int keys[CH], cnt[CH], id[CH];
+      bool include[CH];
+      // histogram seen from u' over the moved state without u'
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        include[c] = (k < m) && k != u;
+        keys[c] = (k == v) ? key_v2 : key[c];
+      }
+      direction_counts<CH>(table, hist, trash, offset, key_u2, keys, include, cnt, id);
+      uint32_t part = 0;
+      int joint_new = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        if (include[c]) part += (uint32_t)(cnt[c] - 1);
+        joint_new += __popcll(__ballot(include[c] && k != v && id[c] == id_u2v2));
+      }
+      // histogram seen from v' over the moved state without v'
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        include[c] = (k < m) && k != v;
+        keys[c] = (k == u) ? key_u2 : key[c];
+      }
+      direction_counts<CH>(table, hist, trash, offset, key_v2, keys, include, cnt, id);
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        if (include[c]) part += (uint32_t)(cnt[c] - 1);
+      // part sums (cnt - 1) over both anchors: twice bad'(u') + bad'(v')
+      const uint32_t twice_bad = wave_sum_dpp(part);
+      const long long new_affected = (long long)(twice_bad >> 1) - joint_new;
+      const long long bad_v = load_value<BT>(a.badness, s * m + v);
+      const long long value = current - (bad_u + bad_v - joint_old) + new_affected;
+      if (lane == 0) out[t] = (OT)value;
+    }
+  }
+}
+
+template <int CH, typename BT, typename OT>
+static int launch_switch(const SwitchArgs& args, hipStream_t stream) {
+  const uint16_t* table = nullptr;
+  DirTableInfo info{};
+  int st = device_dirtable(args.n, &table, &info);
+  if (st) return st;
+  hipDeviceProp_t prop;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (st) return st;
+  int lds_limit = (int)prop.sharedMemPerBlock;
+  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
+  const int table_bytes = table_lds_bytes(info), hist_bytes = info.hist_words * 4;
+  int waves = (lds_limit - table_bytes) / hist_bytes;
+  if (waves > 8) waves = 8;
+  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", args.n);
+  const long long anchors = args.u_fixed >= 0 ? 1 : 2 * args.n;
+  const long long items = args.states * anchors * args.splits;
+  long long blocks = (items + waves - 1) / waves;
+  const long long max_blocks = (long long)prop.multiProcessorCount * 2;
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks < 1) blocks = 1;
+  const size_t lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
+  hipLaunchKernelGGL((switch_scores_kernel<CH, BT, OT>), dim3((unsigned)blocks), dim3(waves * kWave), lds, stream,
+                     args, table, info);
+  return hip_status(hipGetLastError(), "switch_scores_kernel launch");
+}
+
+template <typename BT, typename OT>
+static int dispatch_switch(const SwitchArgs& args, hipStream_t stream) {
+  const int ch = (2 * args.n + kWave - 1) / kWave;
+  switch (ch) {
+    case 1: return launch_switch<1, BT, OT>(args, stream);
+    case 2: return launch_switch<2, BT, OT>(args, stream);
+    case 3: return launch_switch<3, BT, OT>(args, stream);
+    default: return launch_switch<4, BT, OT>(args, stream);
+  }
+}
+
+int switch_status_word(int** word);
+
+static int check_switch(int n, const void* p, const void* q, const void* badness, const void* out) {
+  if (n < 2 || n > kMaxNTable) return set_error(N3_EINVAL, "n must be in [2, %d] (got %d)", kMaxNTable, n);
+  if (!p || !q || !badness || !out) return set_error(N3_EINVAL, "null pointer");
+  return N3_OK;
+}
+
+static int run_switch(SwitchArgs args, bool wide, hipStream_t stream) {
+  int* status = nullptr;
+  int st = switch_status_word(&status);
+  if (st) return st;
+  args.status = status;
+  st = hip_status(hipMemsetAsync(status, 0, sizeof(int), stream), "hipMemsetAsync(status)");
+  if (st) return st;
+  st = wide ? dispatch_switch<long long, long long>(args, stream) : dispatch_switch<int32_t, int32_t>(args, stream);
+  if (st) return st;
+  int host = 0;
+  st = hip_status(hipMemcpyAsync(&host, status, sizeof(int), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(status)");
+  if (st) return st;
+  st = hip_status(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  if (st) return st;
+  if (host & 1) return set_error(N3_EINVAL, "a column lies outside [0, n)");
+  return N3_OK;
+}
+
+int switch_status_word(int** word) {
+  static int* words[64] = {nullptr};
+  int device = 0;
+  (void)hipGetDevice(&device);
+  if (device < 0 || device >= 64) return set_error(N3_ENODEV, "device index %d out of range", device);
+  if (!words[device]) {
+    const int st = hip_status(hipMalloc(&words[device], sizeof(int)), "hipMalloc(status)");
+    if (st) return st;
+  }
+  *word = words[device];
+  return N3_OK;
+}
+
+}  // namespace no3
+
+using namespace no3;
+
+extern "C" int n3_two_switch_scores(const int32_t* p, const int32_t* q, const int32_t* badness, int n, int u,
+                                    int current_score, int32_t* scores, void* stream) {
+  int st = check_switch(n, p, q, badness, scores);
+  if (st) return st;
+  if (u < 0 || u >= 2 * n) return set_error(N3_EINVAL, "selected index %d outside [0, %d)", u, 2 * n);
+  SwitchArgs a{p, q, n, 1, u, -1, 8, badness, nullptr, current_score, scores, 2 * n, nullptr};
+  return run_switch(a, false, as_stream(stream));
+}
+
+extern "C" int n3_evaluate_two_switches(const int32_t* p, const int32_t* q, const int64_t* badness, int n, int u,
+                                        int64_t current_score, int64_t* scores, void* stream) {
+  int st = check_switch(n, p, q, badness, scores);
+  if (st) return st;
+  if (u < 0 || u >= 2 * n) return set_error(N3_EINVAL, "selected index %d outside [0, %d)", u, 2 * n);
+  SwitchArgs a{p, q, n, 1, u, -1, 8, badness, nullptr, (long long)current_score, scores, 2 * n, nullptr};
+  return run_switch(a, true, as_stream(stream));
+}
+
+extern "C" int n3_evaluate_transpositions(const int32_t* p, const int32_t* q, const int64_t* badness, int n, int layer,
+                                          int row, int64_t current_score, int64_t* scores, void* stream) {
+  int st = check_switch(n, p, q, badness, scores);
+  if (st) return st;
+  if (layer != 0 && layer != 1) return set_error(N3_EINVAL, "layer must be 0 (P) or 1 (Q)");
+  if (row < 0 || row >= n) return set_error(N3_EINVAL, "row %d outside [0, %d)", row, n);
+  SwitchArgs a{p, q, n, 1, 2 * row + layer, layer, 8, badness, nullptr, (long long)current_score, scores, n, nullptr};
+  return run_switch(a, true, as_stream(stream));
+}
+
+extern "C" int n3_delta_table(const int32_t* p, const int32_t* q, const int32_t* scores, const int32_t* badness, int n,
+                              int64_t count, int32_t* table, void* stream) {
+  int st = check_switch(n, p, q, badness, table);
+  if (st) return st;
+  if (!scores) return set_error(N3_EINVAL, "null pointer");
+  if (count < 0) return set_error(N3_EINVAL, "negative count");
+  if (count == 0) return N3_OK;
+  SwitchArgs a{p, q, n, (long long)count, -1, -1, 1, badness, scores, 0, table, 2 * n, nullptr};
+  return run_switch(a, false, as_stream(stream));
+}

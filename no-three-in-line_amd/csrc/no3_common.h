@@ -1,0 +1,91 @@
+// Shared host/device definitions of the no-three-in-line gfx950 engine.
+//
+// Geometry conventions (the reference's, accelerated.py:20-24, gpu.py:20-21):
+// slot s of a state holds the point (row = s >> 1, column = P[row] if s is
+// even, Q[row] if odd); a state has m = 2n slots.
+//
+// Direction table.  Two points a, b differ by (dr, dc) with |dr|, |dc| <= n-1.
+// A point is encoded as key = row * W + column with W = 2n - 1, so
+// key_b - key_a = dr * W + dc identifies (dr, dc) uniquely, and
+// dir_table[key_b - key_a + (n-1) * W + (n-1)] is the id of the primitive
+// undirected direction of (dr, dc) (verifier_b.py:8-19: divide by the gcd,
+// flip the sign so that dr > 0 or dr == 0 < dc).  Ids are dense in
+// [0, D(n)); the zero vector (coincident points) has id D(n).  Two points b,
+// c are collinear with a iff their ids seen from a are equal, so the number
+// of collinear triples through a is sum_d C(cnt_d, 2) over the histogram of
+// ids seen from a (the counting form of verifier_b.find_collinear_triple).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace no3 {
+
+constexpr int kWave = 64;
+constexpr int kMaxN = 256;          // determinant paths: m <= 512
+constexpr int kMaxNTable = 100;     // direction-table paths: W*W*2 bytes of LDS
+constexpr int kSentinel32 = 1 << 30;           // gpu_delta.py:30
+constexpr long long kSentinel64 = 1LL << 60;   // accelerated.py:58,140
+
+// Philox stream tags (counter word 3).
+constexpr uint32_t kTagGen = 0x47454E53u;   // state generator
+constexpr uint32_t kTagMove = 0x4D4F5645u;  // one tempering proposal
+constexpr uint32_t kTagBias = 0x42494153u;  // badness-biased slot draw
+constexpr uint32_t kTagExch = 0x45584348u;  // replica exchange
+
+struct DirTableInfo {
+  int n;
+  int width;        // W = 2n - 1
+  int offset;       // (n-1) * W + (n-1)
+  int entries;      // W * W
+  int directions;   // D(n); the zero vector has id D(n)
+  int hist_words;   // 32-bit words of one byte-packed histogram: the D+1
+                    // byte counters, then a spare "trash" word that masked
+                    // lanes update instead of branching around the update
+};
+
+__host__ __device__ inline int trash_word(const DirTableInfo& info) { return info.hist_words - 1; }
+
+__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
+  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                              uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+// Orders LDS traffic of the lanes of one wavefront (LDS executes a wave's
+// instructions in issue order; this keeps the compiler from moving them).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline int wave_sum(int x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+
+__device__ inline int lane_id() { return (int)__lane_id(); }
+
+}  // namespace no3

@@ -1,0 +1,443 @@
+// Population kernels: the seeded state generator and exact collinear-triple
+// scoring with per-slot badness.
+//
+// Scoring (direction histograms).  One wavefront owns one state; lane l holds
+// slots l, l+64, l+128, ... in registers as keys row*W + column.  For every
+// anchor a, the lanes look up the direction id of (slot - a) in the LDS
+// direction table and add one to a byte-packed LDS histogram of ids; every
+// lane then reads back the count of its own id, cnt_a(b) = number of points
+// on the line (a, b) seen from a, and clears its counter.  A slot b lies in
+// sum_{a != b} (cnt_a(b) - 1) / 2 collinear triples (each triple {a, b, c}
+// is seen from a and from c), which is the reference's per-point badness
+// (accelerated.py:34-36) and E = sum_b badness_b / 3 (accelerated.py:33).
+// Work per state is m(m-1) table lookups instead of C(m,3) determinants.
+//
+// States with coincident points (P[r] == Q[r]) have no direction for the
+// pair; those states take the determinant path, which counts them exactly
+// like the reference (gpu.py:19-33).
+#include "runtime.h"
+#include "wave_ops.h"
+
+namespace no3 {
+
+// ---------------------------------------------------------------------------
+// generator
+// ---------------------------------------------------------------------------
+
+constexpr int kGenThreads = 64;
+constexpr int kGenStride = 2 * kMaxN + 4;  // bytes per thread in LDS
+
+struct GenStream {
+  uint32_t k0, k1, c0, c1, c2;
+  uint32_t buf[4];
+  int pos;
+  __device__ uint32_t draw() {
+    if (pos == 4) {
+      philox4x32_10(c0, c1, c2, kTagGen, k0, k1, buf);
+      c2 += 1u;
+      pos = 0;
+    }
+    return buf[pos++];
+  }
+};
+
+__global__ void __launch_bounds__(kGenThreads) generate_kernel(uint64_t seed, int n, long long count,
+                                                               long long first_index, int32_t* __restrict__ p,
+                                                               int32_t* __restrict__ q) {
+  __shared__ uint8_t space[kGenThreads * kGenStride];
+  const long long s = (long long)blockIdx.x * kGenThreads + threadIdx.x;
+  if (s >= count) return;
+  uint8_t* P = space + threadIdx.x * kGenStride;
+  uint8_t* Q = P + kMaxN;
+  const long long index = first_index + s;
+  GenStream g{(uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)index, (uint32_t)((unsigned long long)index >> 32), 0u,
+              {0, 0, 0, 0}, 4};
+  for (int i = 0; i < n; ++i) {
+    P[i] = (uint8_t)i;
+    Q[i] = (uint8_t)i;
+  }
+  for (int i = n - 1; i >= 1; --i) {
+    const int j = (int)mulhi32(g.draw(), (uint32_t)(i + 1));
+    const uint8_t t = P[i];
+    P[i] = P[j];
+    P[j] = t;
+  }
+  for (int i = n - 1; i >= 1; --i) {
+    const int j = (int)mulhi32(g.draw(), (uint32_t)(i + 1));
+    const uint8_t t = Q[i];
+    Q[i] = Q[j];
+    Q[j] = t;
+  }
+  for (int i = 0; i < n; ++i) {
+    if (Q[i] == P[i]) {
+      int r = (int)mulhi32(g.draw(), (uint32_t)(n - 1));
+      if (r >= i) r += 1;
+      const uint8_t t = Q[i];
+      Q[i] = Q[r];
+      Q[r] = t;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    p[s * n + i] = P[i];
+    q[s * n + i] = Q[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// determinant path (one wavefront per state)
+// ---------------------------------------------------------------------------
+
+// Exact reference count for the state whose columns are in cols[0..m) (LDS):
+// for every pair i < j (uniform), lanes test the slots k > j with
+// dr1 * (c_k - c_i) == dc1 * (r_k - r_i) (accelerated.py:32, gpu.py:30).
+// Hits for k stay in the lane's register; hits for i and j are credited in
+// the LDS array credit[0..m).  Returns, per lane and chunk, the badness.
+template <int CH>
+__device__ void determinant_state(const int32_t* cols, int32_t* credit, int m, int bad[CH]) {
+  const int lane = lane_id();
+  int colk[CH], rowk[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    colk[c] = k < m ? cols[k] : 0;
+    rowk[c] = k >> 1;
+    bad[c] = 0;
+  }
+  for (int i = lane; i < m; i += kWave) credit[i] = 0;
+  wave_sync();
+  for (int i = 0; i < m - 2; ++i) {
+    const int ri = i >> 1;
+    const int ci = __builtin_amdgcn_readfirstlane(cols[i]);
+    for (int j = i + 1; j < m - 1; ++j) {
+      const int dr1 = (j >> 1) - ri;
+      const int dc1 = __builtin_amdgcn_readfirstlane(cols[j]) - ci;
+      const int key = dr1 * ci - dc1 * ri;
+      int hits = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        if (kWave * c + kWave - 1 <= j) continue;  // whole chunk at k <= j
+        const int k = lane + kWave * c;
+        const bool h = (k > j) && (k < m) && (__mul24(dr1, colk[c]) - __mul24(dc1, rowk[c]) == key);
+        bad[c] += h ? 1 : 0;
+        hits += __popcll(__ballot(h));
+      }
+      if (hits) {
+        if (lane == 0) {
+          credit[i] += hits;
+          credit[j] += hits;
+        }
+        wave_sync();
+      }
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    if (k < m) bad[c] += credit[k];
+  }
+  wave_sync();
+}
+
+template <int CH>
+__global__ void __launch_bounds__(256) score_determinant_kernel(const int32_t* __restrict__ p,
+                                                                const int32_t* __restrict__ q, int n, long long count,
+                                                                int32_t* __restrict__ scores,
+                                                                int32_t* __restrict__ badness,
+                                                                int* __restrict__ status) {
+  __shared__ int32_t space[4][2 * 2 * kMaxN];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  int32_t* cols = space[wave];
+  int32_t* credit = cols + 2 * kMaxN;
+  const int m = 2 * n;
+  for (long long s = (long long)blockIdx.x * waves + wave; s < count; s += (long long)gridDim.x * waves) {
+    int outside = 0;
+    for (int k = lane; k < m; k += kWave) {
+      const int col = (k & 1) ? q[s * n + (k >> 1)] : p[s * n + (k >> 1)];
+      outside |= (col < 0 || col >= n);
+      cols[k] = col;
+    }
+    wave_sync();
+    if (__any(outside)) {
+      if (lane == 0) {
+        scores[s] = -1;
+        atomicOr(status, 1);
+      }
+      continue;
+    }
+    int bad[CH];
+    determinant_state<CH>(cols, credit, m, bad);
+    int sum = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      if (k < m) {
+        sum += bad[c];
+        if (badness) badness[s * m + k] = bad[c];
+      }
+    }
+    sum = wave_sum(sum);
+    if (lane == 0) scores[s] = sum / 3;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// direction-histogram path (one wavefront per state)
+// ---------------------------------------------------------------------------
+
+template <int CH>
+__global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __restrict__ p, const int32_t* __restrict__ q,
+                                                          int n, long long count, int32_t* __restrict__ scores,
+                                                          int32_t* __restrict__ badness,
+                                                          const uint16_t* __restrict__ table_g, DirTableInfo info,
+                                                          int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  stage_table(table_g, table, info.entries);
+  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+
+  const int m = 2 * n;
+  const int width = info.width;
+  for (long long s = (long long)blockIdx.x * waves + wave; s < count; s += (long long)gridDim.x * waves) {
+    int key[CH], acc[CH];
+    int outside = 0, coincide = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      acc[c] = 0;
+      key[c] = 0;
+      if (k < m) {
+        const int col = (k & 1) ? q[s * n + (k >> 1)] : p[s * n + (k >> 1)];
+        outside |= (col < 0 || col >= n);
+        key[c] = (k >> 1) * width + col;
+      }
+      // slots 2r and 2r+1 sit in neighbouring lanes of the same chunk
+      const int partner = __shfl_xor(key[c], 1, kWave);
+      coincide |= (k < m) && (partner == key[c]);
+    }
+    if (__any(outside)) {
+      if (lane == 0) {
+        scores[s] = -1;
+        atomicOr(status, 1);
+      }
+      continue;
+    }
+    if (__any(coincide)) {
+      // coincident points: exact determinant count, with the histogram area
+      // as scratch for the state's columns (it is cleared again below)
+      int32_t* cols = reinterpret_cast<int32_t*>(hist);
+      int32_t* credit = cols + m;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        if (k < m) cols[k] = key[c] - (k >> 1) * width;
+      }
+      wave_sync();
+      int bad[CH];
+      determinant_state<CH>(cols, credit, m, bad);
+      int sum = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        if (k < m) {
+          sum += bad[c];
+          if (badness) badness[s * m + k] = bad[c];
+        }
+      }
+      sum = wave_sum(sum);
+      if (lane == 0) scores[s] = sum / 3;
+      for (int i = lane; i < 2 * m; i += kWave) hist[i] = 0;
+      wave_sync();
+      continue;
+    }
+
+    // lanes past the last slot carry a valid key (slot 0's) and count into
+    // the trash word, so that every lane runs every instruction
+    const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (lane + kWave * c >= m) key[c] = key0;
+    const int trash = trash_word(info);
+#pragma unroll
+    for (int ca = 0; ca < CH; ++ca) {
+      const int limit = min(kWave, m - kWave * ca);
+      for (int l = 0; l < limit; ++l) {
+        const int base = info.offset - __builtin_amdgcn_readlane(key[ca], l);
+        uint32_t id[CH], word[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) id[c] = table[key[c] + base];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          word[c] = (lane + kWave * c < m) ? (id[c] >> 2) : (uint32_t)trash;
+          atomicAdd(&hist[word[c]], 1u << ((id[c] << 3) & 31u));
+        }
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] += (int)((hist[word[c]] >> ((id[c] << 3) & 31u)) & 0xFFu);
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
+        wave_sync();
+      }
+    }
+    // acc = 1 (own zero-vector bin) + sum over the m-1 other anchors of cnt
+    int sum = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      if (k < m) {
+        const int bad = (acc[c] - m) >> 1;
+        sum += bad;
+        if (badness) badness[s * m + k] = bad;
+      }
+    }
+    sum = wave_sum(sum);
+    if (lane == 0) scores[s] = sum / 3;
+  }
+}
+
+template <int CH>
+static int launch_slope(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
+                        const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
+  hipDeviceProp_t prop;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  int st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (st) return st;
+  int lds_limit = (int)prop.sharedMemPerBlock;
+  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
+  const int table_bytes = table_lds_bytes(info);
+  const int hist_bytes = info.hist_words * 4;
+  int waves = (lds_limit - table_bytes) / hist_bytes;
+  if (waves > 16) waves = 16;
+  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
+  const size_t lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
+  // the LDS budget admits one block per CU; the blocks stride over the states
+  long long blocks = ((long long)count + waves - 1) / waves;
+  const long long max_blocks = (long long)prop.multiProcessorCount;
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(score_slope_kernel<CH>, dim3((unsigned)blocks), dim3(waves * kWave), lds, stream, p, q, n,
+                     (long long)count, scores, badness, table, info, status);
+  return hip_status(hipGetLastError(), "score_slope_kernel launch");
+}
+
+template <int CH>
+static int launch_determinant(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores,
+                              int32_t* badness, int* status, hipStream_t stream) {
+  long long blocks = ((long long)count + 3) / 4;
+  if (blocks > 65535LL * 4) blocks = 65535LL * 4;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(score_determinant_kernel<CH>, dim3((unsigned)blocks), dim3(256), 0, stream, p, q, n,
+                     (long long)count, scores, badness, status);
+  return hip_status(hipGetLastError(), "score_determinant_kernel launch");
+}
+
+// Per-device status word, cleared before and read after a scoring launch.
+static int* status_word(int* st_out) {
+  static int* words[64] = {nullptr};
+  int device = 0;
+  (void)hipGetDevice(&device);
+  if (device < 0 || device >= 64) {
+    *st_out = set_error(N3_ENODEV, "device index %d out of range", device);
+    return nullptr;
+  }
+  if (!words[device]) {
+    int* w = nullptr;
+    *st_out = hip_status(hipMalloc(&w, sizeof(int)), "hipMalloc(status)");
+    if (*st_out) return nullptr;
+    *st_out = hip_status(hipMemset(w, 0, sizeof(int)), "hipMemset(status)");
+    if (*st_out) return nullptr;
+    words[device] = w;
+  }
+  *st_out = N3_OK;
+  return words[device];
+}
+
+int score_population_impl(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
+                          int method, hipStream_t stream) {
+  int st = check_columns_launch_args(n, count);
+  if (st) return st;
+  if (count == 0) return N3_OK;
+  if (!p || !q || !scores) return set_error(N3_EINVAL, "null pointer");
+  // the kernels only ever set bits of the status word; n3_score_status
+  // reads and clears it, so a launch costs no extra device operation
+  int* status = status_word(&st);
+  if (st) return st;
+  const int m = 2 * n;
+  const int ch = (m + kWave - 1) / kWave;
+  if (method == N3_SCORE_DETERMINANT || n > kMaxNTable) {
+    switch (ch) {
+      case 1: st = launch_determinant<1>(p, q, n, count, scores, badness, status, stream); break;
+      case 2: st = launch_determinant<2>(p, q, n, count, scores, badness, status, stream); break;
+      case 3: st = launch_determinant<3>(p, q, n, count, scores, badness, status, stream); break;
+      case 4: st = launch_determinant<4>(p, q, n, count, scores, badness, status, stream); break;
+      case 5: st = launch_determinant<5>(p, q, n, count, scores, badness, status, stream); break;
+      case 6: st = launch_determinant<6>(p, q, n, count, scores, badness, status, stream); break;
+      case 7: st = launch_determinant<7>(p, q, n, count, scores, badness, status, stream); break;
+      default: st = launch_determinant<8>(p, q, n, count, scores, badness, status, stream); break;
+    }
+  } else if (method == N3_SCORE_AUTO || method == N3_SCORE_SLOPE) {
+    const uint16_t* table = nullptr;
+    DirTableInfo info{};
+    st = device_dirtable(n, &table, &info);
+    if (st) return st;
+    switch (ch) {
+      case 1: st = launch_slope<1>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 2: st = launch_slope<2>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 3: st = launch_slope<3>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      default: st = launch_slope<4>(p, q, n, count, scores, badness, table, info, status, stream); break;
+    }
+  } else {
+    return set_error(N3_EINVAL, "unknown scoring method %d", method);
+  }
+  if (st) return st;
+  return N3_OK;
+}
+
+// Reads back the status word of the last scoring launch on this device
+// (synchronises `stream`).
+int score_status(hipStream_t stream) {
+  int st = N3_OK;
+  int* status = status_word(&st);
+  if (st) return st;
+  int host = 0;
+  st = hip_status(hipMemcpyAsync(&host, status, sizeof(int), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(status)");
+  if (st) return st;
+  st = hip_status(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  if (st) return st;
+  if (host == 0) return N3_OK;
+  st = hip_status(hipMemsetAsync(status, 0, sizeof(int), stream), "hipMemsetAsync(status)");
+  if (st) return st;
+  st = hip_status(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  if (st) return st;
+  return set_error(N3_EINVAL, "a column lies outside [0, n)");
+}
+
+}  // namespace no3
+
+using namespace no3;
+
+extern "C" int n3_generate_states(uint64_t seed, int n, int64_t count, int64_t first_index, int32_t* p, int32_t* q,
+                                  void* stream) {
+  int st = check_columns_launch_args(n, count);
+  if (st) return st;
+  if (count == 0) return N3_OK;
+  if (!p || !q) return set_error(N3_EINVAL, "null pointer");
+  const long long blocks = ((long long)count + kGenThreads - 1) / kGenThreads;
+  hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(kGenThreads), 0, as_stream(stream), seed, n,
+                     (long long)count, (long long)first_index, p, q);
+  return hip_status(hipGetLastError(), "generate_kernel launch");
+}
+
+extern "C" int n3_score_population(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores,
+                                   int32_t* badness_or_null, int method, void* stream) {
+  return score_population_impl(p, q, n, count, scores, badness_or_null, method, as_stream(stream));
+}
+
+extern "C" int n3_score_status(void* stream) { return score_status(as_stream(stream)); }

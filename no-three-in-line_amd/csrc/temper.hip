@@ -1,0 +1,563 @@
+// Replica-exchange tempering (tempering.py:19-225) on the device.
+//
+// walk: one wavefront per replica slot keeps the slot's state in registers
+// (lane l holds slots l, l+64, ...: key row*W + column and its badness) and
+// runs `steps` proposals of the reference's walk kernel (tempering.py:48-110)
+// without leaving the kernel.  The exact energy change of a 2-switch (u, v)
+// comes from four direction histograms in LDS instead of the reference's
+// O(m^2) determinant pair loops:
+//
+//   delta = [bad'(u') + bad'(v') - joint'(u',v')] - [bad(u) + bad(v) - joint(u,v)]
+//
+// bad'(x) = sum over the histogram seen from x of C(cnt, 2); joint counts the
+// points on the line through both moved points.  On acceptance the badness of
+// every other slot k changes by
+//
+//   (cnt_u'(k) + cnt_v'(k) - [k on u'v']) - (cnt_u(k) + cnt_v(k) - [k on uv])
+//
+// where cnt_x(k) is the number of points on the line (x, k) seen from x (the
+// slot-level form of the db[] accumulation of tempering.py:71-88).
+//
+// exchange: adjacent-temperature swaps inside each ladder
+// (tempering.py:169-191), one wavefront per candidate pair; an accepted pair
+// swaps the slots' columns, badness, scores and removed counters.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "runtime.h"
+#include "wave_ops.h"
+
+namespace no3 {
+
+struct WalkArgs {
+  int n;
+  int replicas;
+  int steps;
+  int bad_bias;
+  uint32_t seed_lo, seed_hi;
+  int32_t* columns;
+  int32_t* badness;
+  int32_t* scores;
+  int32_t* best_columns;
+  int32_t* best_scores;
+  int32_t* accepted;
+  const int32_t* reference;
+  int32_t* removed;
+  const uint32_t* thr;
+  int k_move;
+  const uint32_t* counters;
+  unsigned long long* proposals;
+};
+
+template <int CH>
+__global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* __restrict__ table_g,
+                                                   DirTableInfo info) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  stage_table(table_g, table, info.entries);
+  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+
+  const int slot = blockIdx.x * waves + wave;
+  if (slot >= a.replicas) return;
+  const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
+  const uint32_t sweep = a.counters[0];
+  const long long base = (long long)slot * m;
+
+  int key[CH], bad[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    key[c] = k < m ? (k >> 1) * width + a.columns[base + k] : 0;
+    bad[c] = k < m ? a.badness[base + k] : 0;
+  }
+  // lanes past the last slot hold slot 0's key (always a valid point)
+  {
+    const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (lane + kWave * c >= m) key[c] = key0;
+  }
+  const int trash = trash_word(info);
+  int score = a.scores[slot];
+  int removed = a.removed[slot];
+  int best = a.best_scores[slot];
+  int accepted = 0;
+  unsigned long long valid_moves = 0;
+  const uint32_t* thr = a.thr + (long long)slot * a.k_move;
+
+  for (int step = 0; step < a.steps; ++step) {
+    uint32_t r[4];
+    philox4x32_10((uint32_t)step, sweep, (uint32_t)slot, kTagMove, a.seed_lo, a.seed_hi, r);
+    int u = (int)mulhi32(r[0], (uint32_t)m);
+    if (a.bad_bias > 0 && score > 0 && (int)mulhi32(r[3], 100u) < a.bad_bias) {
+      // tempering.py:51-55: u drawn in proportion to badness
+      uint32_t rb[4];
+      philox4x32_10((uint32_t)step, sweep, (uint32_t)slot, kTagBias, a.seed_lo, a.seed_hi, rb);
+      const int target = (int)mulhi32(rb[0], (uint32_t)(3 * score));
+      int below = 0, carry = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        // inclusive prefix of badness over the slots of this chunk
+        int x = bad[c];
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const int y = __shfl_up(x, off, kWave);
+          if (lane >= off) x += y;
+        }
+        const int k = lane + kWave * c;
+        below += __popcll(__ballot(k < m && carry + x <= target));
+        carry += __builtin_amdgcn_readlane(x, 63);
+      }
+      u = below < m - 1 ? below : m - 1;
+    }
+    const int v = (int)mulhi32(r[1], (uint32_t)m);
+    const int ru = u >> 1, rv = v >> 1;
+    const int key_u = slot_value<CH>(key, u), key_v = slot_value<CH>(key, v);
+    const int cu = key_u - ru * width, cv = key_v - rv * width;
+    const int cu_other = slot_value<CH>(key, u ^ 1) - ru * width;
+    const int cv_other = slot_value<CH>(key, v ^ 1) - rv * width;
+    const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
+    if (!valid) continue;
+    valid_moves += 1;
+    const int ur = 2 * ru, vr = 2 * rv;
+    const int dd = (cu == a.reference[ur] || cu == a.reference[ur + 1]) +
+                   (cv == a.reference[vr] || cv == a.reference[vr + 1]) -
+                   (cv == a.reference[ur] || cv == a.reference[ur + 1]) -
+                   (cu == a.reference[vr] || cu == a.reference[vr + 1]);
+
+    // joint(u, v) in the current state
+    const int id_uv = table[key_v - key_u + offset];
+    int idu[CH];
+    bool others[CH];
+    uint64_t on_uv[CH];
+    int joint_old = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      others[c] = (k < m) && k != u && k != v;
+      idu[c] = table[key[c] - key_u + offset];
+      on_uv[c] = __ballot(others[c] && idu[c] == id_uv);
+      joint_old += __popcll(on_uv[c]);
+    }
+    // histograms seen from u' and v' in the moved state
+    const int key_u2 = ru * width + cv, key_v2 = rv * width + cu;
+    const int id_u2v2 = table[key_v2 - key_u2 + offset];
+    int keys[CH], cnt_u2[CH], cnt_v2[CH], id[CH];
+    bool include[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      include[c] = (k < m) && k != u;
+      keys[c] = (k == v) ? key_v2 : key[c];
+    }
+    direction_counts<CH>(table, hist, trash, offset, key_u2, keys, include, cnt_u2, id);
+    uint32_t part = 0;
+    int joint_new = 0;
+    uint64_t on_u2v2[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (include[c]) part += (uint32_t)(cnt_u2[c] - 1);
+      on_u2v2[c] = __ballot(others[c] && id[c] == id_u2v2);
+      joint_new += __popcll(on_u2v2[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      include[c] = (k < m) && k != v;
+      keys[c] = (k == u) ? key_u2 : key[c];
+    }
+    direction_counts<CH>(table, hist, trash, offset, key_v2, keys, include, cnt_v2, id);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (include[c]) part += (uint32_t)(cnt_v2[c] - 1) << 16;
+    // twice bad'(u') in the low half, twice bad'(v') in the high half
+    const uint32_t packed = wave_sum_dpp(part);
+    const int bad_u2 = (int)((packed & 0xFFFFu) >> 1), bad_v2 = (int)((packed >> 16) >> 1);
+    const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
+    const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
+
+    // Metropolis on the integer delta (tempering.py:91-95)
+    bool accept = delta <= 0;
+    if (!accept) accept = r[2] < thr[delta < a.k_move ? delta : a.k_move - 1];
+    if (!accept) continue;
+
+    // badness of the unmoved slots: histograms seen from u and v now
+    int cnt_u[CH], cnt_v[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
+    direction_counts_known<CH>(hist, trash, idu, include, cnt_u);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+    direction_counts<CH>(table, hist, trash, offset, key_v, key, include, cnt_v, id);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      const int bit_old = (int)((on_uv[c] >> lane) & 1ull);
+      const int bit_new = (int)((on_u2v2[c] >> lane) & 1ull);
+      if (others[c]) bad[c] += (cnt_u2[c] + cnt_v2[c] - bit_new) - (cnt_u[c] + cnt_v[c] - bit_old);
+      if (k == u) {
+        bad[c] = bad_u2;
+        key[c] = key_u2;
+      }
+      if (k == v) {
+        bad[c] = bad_v2;
+        key[c] = key_v2;
+      }
+    }
+    score += delta;
+    removed += dd;
+    accepted += 1;
+    if (score < best) {
+      best = score;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        if (k < m) a.best_columns[base + k] = key[c] - (k >> 1) * width;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    if (k < m) {
+      a.columns[base + k] = key[c] - (k >> 1) * width;
+      a.badness[base + k] = bad[c];
+    }
+  }
+  if (lane == 0) {
+    a.scores[slot] = score;
+    a.removed[slot] = removed;
+    a.best_scores[slot] = best;
+    a.accepted[slot] += accepted;
+    atomicAdd(&a.proposals[0], valid_moves);
+    atomicAdd(&a.proposals[1], (unsigned long long)accepted);
+  }
+}
+
+struct ExchangeArgs {
+  int n;
+  int replicas;
+  int ladder;
+  uint32_t seed_lo, seed_hi;
+  int32_t* columns;
+  int32_t* badness;
+  int32_t* scores;
+  int32_t* removed;
+  const uint32_t* thr;
+  const int8_t* sign;
+  int k_exchange;
+  uint32_t* counters;
+};
+
+// One wavefront per candidate pair (a, a+1); pairs start at round % 2 inside
+// every ladder and do not overlap, so the swaps need no synchronisation.
+__global__ void __launch_bounds__(256) exchange_kernel(ExchangeArgs x) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  const uint32_t round = x.counters[1];
+  const int per_ladder = x.ladder / 2;  // candidate pairs per ladder (upper bound)
+  const int ladders = x.replicas / x.ladder;
+  if (wave >= ladders * per_ladder) return;
+  const int ladder = wave / per_ladder;
+  const int start = ladder * x.ladder;
+  const int a = start + (int)(round & 1u) + 2 * (wave % per_ladder);
+  if (a >= start + x.ladder - 1) return;
+  const int b = a + 1;
+  const long long diff = (long long)x.scores[a] - (long long)x.scores[b];
+  const int sg = x.sign[a];
+  bool accept;
+  if (sg == 0 || (sg > 0 && diff >= 0) || (sg < 0 && diff <= 0)) {
+    accept = true;
+  } else {
+    uint32_t r[4];
+    philox4x32_10((uint32_t)a, round, (uint32_t)ladder, kTagExch, x.seed_lo, x.seed_hi, r);
+    long long d = diff < 0 ? -diff : diff;
+    if (d > x.k_exchange - 1) d = x.k_exchange - 1;
+    accept = r[0] < x.thr[(long long)a * x.k_exchange + d];
+  }
+  if (!accept) return;
+  const int m = 2 * x.n;
+  for (int i = lane; i < m; i += kWave) {
+    const long long ia = (long long)a * m + i, ib = (long long)b * m + i;
+    const int32_t ca = x.columns[ia], cb = x.columns[ib];
+    x.columns[ia] = cb;
+    x.columns[ib] = ca;
+    const int32_t ba = x.badness[ia], bb = x.badness[ib];
+    x.badness[ia] = bb;
+    x.badness[ib] = ba;
+  }
+  if (lane == 0) {
+    const int32_t sa = x.scores[a];
+    x.scores[a] = x.scores[b];
+    x.scores[b] = sa;
+    const int32_t ra = x.removed[a];
+    x.removed[a] = x.removed[b];
+    x.removed[b] = ra;
+    atomicAdd(reinterpret_cast<unsigned long long*>(x.counters + 2), 1ull);
+  }
+}
+
+__global__ void bump_kernel(uint32_t* counters, uint32_t d_sweep, uint32_t d_round) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    counters[0] += d_sweep;
+    counters[1] += d_round;
+  }
+}
+
+static int check_chains(const n3_chains* c) {
+  if (!c) return set_error(N3_EINVAL, "null chains");
+  if (c->n < 2 || c->n > kMaxNTable) return set_error(N3_EINVAL, "n must be in [2, %d] (got %d)", kMaxNTable, c->n);
+  if (c->replicas < 1) return set_error(N3_EINVAL, "replicas must be positive");
+  if (!c->columns || !c->badness || !c->scores || !c->best_columns || !c->best_scores || !c->accepted ||
+      !c->reference || !c->removed_points || !c->move_thresholds || !c->counters || !c->proposals)
+    return set_error(N3_EINVAL, "null chains array");
+  if (c->k_move < 2) return set_error(N3_EINVAL, "k_move must be at least 2");
+  return N3_OK;
+}
+
+struct WalkPlan {
+  const uint16_t* table;
+  DirTableInfo info;
+  int waves;
+  unsigned blocks;
+  size_t lds;
+  void (*kernel)(WalkArgs, const uint16_t*, DirTableInfo);
+};
+
+static int walk_plan(int n, int replicas, WalkPlan* plan) {
+  int st = device_dirtable(n, &plan->table, &plan->info);
+  if (st) return st;
+  hipDeviceProp_t prop;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (st) return st;
+  int lds_limit = (int)prop.sharedMemPerBlock;
+  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
+  const int table_bytes = table_lds_bytes(plan->info), hist_bytes = plan->info.hist_words * 4;
+  int waves = (lds_limit - table_bytes) / hist_bytes;
+  if (waves > 16) waves = 16;
+  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
+  // spread small populations over more CUs
+  const int per_cu = (replicas + prop.multiProcessorCount - 1) / prop.multiProcessorCount;
+  if (waves > per_cu) waves = per_cu < 1 ? 1 : per_cu;
+  plan->waves = waves;
+  plan->blocks = (unsigned)((replicas + waves - 1) / waves);
+  plan->lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
+  const int ch = (2 * n + kWave - 1) / kWave;
+  switch (ch) {
+    case 1: plan->kernel = walk_kernel<1>; break;
+    case 2: plan->kernel = walk_kernel<2>; break;
+    case 3: plan->kernel = walk_kernel<3>; break;
+    default: plan->kernel = walk_kernel<4>; break;
+  }
+  return N3_OK;
+}
+
+static WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed) {
+  return WalkArgs{c->n, c->replicas, steps, bad_bias, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
+                  c->scores, c->best_columns, c->best_scores, c->accepted, c->reference, c->removed_points,
+                  c->move_thresholds, c->k_move, c->counters, reinterpret_cast<unsigned long long*>(c->proposals)};
+}
+
+static int enqueue_advance(const n3_chains* c, const WalkPlan& plan, int steps, int bad_bias, uint64_t seed,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream,
+                     walk_args(c, steps, bad_bias, seed), plan.table, plan.info);
+  int st = hip_status(hipGetLastError(), "walk_kernel launch");
+  if (st) return st;
+  hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, 1u, 0u);
+  return hip_status(hipGetLastError(), "bump_kernel launch");
+}
+
+static int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream) {
+  ExchangeArgs x{c->n, c->replicas, ladder, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
+                 c->scores, c->removed_points, c->exchange_thresholds, c->exchange_sign, c->k_exchange, c->counters};
+  const int pairs = (c->replicas / ladder) * (ladder / 2);
+  if (pairs > 0) {
+    const int threads = 256, per_block = threads / kWave;
+    hipLaunchKernelGGL(exchange_kernel, dim3((unsigned)((pairs + per_block - 1) / per_block)), dim3(threads), 0, stream,
+                       x);
+    int st = hip_status(hipGetLastError(), "exchange_kernel launch");
+    if (st) return st;
+  }
+  hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, 0u, 1u);
+  return hip_status(hipGetLastError(), "bump_kernel launch");
+}
+
+static int check_exchange(const n3_chains* c, int ladder) {
+  if (ladder < 2 || c->replicas % ladder) return set_error(N3_EINVAL, "ladder_size must divide replica count");
+  if (!c->exchange_thresholds || !c->exchange_sign || c->k_exchange < 2)
+    return set_error(N3_EINVAL, "exchange thresholds missing");
+  return N3_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph cache for n3_temper_run
+// ---------------------------------------------------------------------------
+
+struct GraphKey {
+  n3_chains chains;
+  int steps, ladder, bad_bias, sweeps_per_graph;
+  uint64_t seed;
+  hipStream_t stream;
+  int device;
+};
+
+static bool same_key(const GraphKey& x, const GraphKey& y) {
+  return std::memcmp(&x.chains, &y.chains, sizeof(n3_chains)) == 0 && x.steps == y.steps && x.ladder == y.ladder &&
+         x.bad_bias == y.bad_bias && x.sweeps_per_graph == y.sweeps_per_graph && x.seed == y.seed &&
+         x.stream == y.stream && x.device == y.device;
+}
+
+struct GraphEntry {
+  GraphKey key;
+  hipGraphExec_t exec;
+  hipGraph_t graph;
+};
+
+static std::mutex g_graph_mutex;
+static std::vector<GraphEntry> g_graphs;
+
+static int capture_graph(const n3_chains* c, const WalkPlan& plan, int sweeps, int steps, int ladder, int bad_bias,
+                         uint64_t seed, hipGraph_t* graph, hipGraphExec_t* exec) {
+  hipStream_t cap = nullptr;
+  int st = hip_status(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+  if (st) return st;
+  st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (!st) {
+    for (int i = 0; i < sweeps && !st; ++i) {
+      st = enqueue_advance(c, plan, steps, bad_bias, seed, cap);
+      if (!st && ladder > 1) st = enqueue_exchange(c, ladder, seed, cap);
+    }
+    hipGraph_t g = nullptr;
+    const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
+    if (!st) st = end;
+    if (!st) {
+      st = hip_status(hipGraphInstantiate(exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+      if (st)
+        (void)hipGraphDestroy(g);
+      else
+        *graph = g;
+    } else if (g) {
+      (void)hipGraphDestroy(g);
+    }
+  }
+  (void)hipStreamDestroy(cap);
+  return st;
+}
+
+}  // namespace no3
+
+using namespace no3;
+
+extern "C" int n3_temper_advance(const n3_chains* chains, int steps, int bad_bias, uint64_t seed, void* stream) {
+  int st = check_chains(chains);
+  if (st) return st;
+  if (steps < 1 || bad_bias < 0 || bad_bias > 100)
+    return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
+  WalkPlan plan;
+  st = walk_plan(chains->n, chains->replicas, &plan);
+  if (st) return st;
+  return enqueue_advance(chains, plan, steps, bad_bias, seed, as_stream(stream));
+}
+
+extern "C" int n3_temper_exchange(const n3_chains* chains, int ladder, uint64_t seed, void* stream) {
+  int st = check_chains(chains);
+  if (st) return st;
+  st = check_exchange(chains, ladder);
+  if (st) return st;
+  return enqueue_exchange(chains, ladder, seed, as_stream(stream));
+}
+
+extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int ladder, int bad_bias, uint64_t seed,
+                             void* stream) {
+  int st = check_chains(chains);
+  if (st) return st;
+  if (sweeps < 0) return set_error(N3_EINVAL, "negative sweeps");
+  if (steps < 1 || bad_bias < 0 || bad_bias > 100)
+    return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
+  if (ladder > 1) {
+    st = check_exchange(chains, ladder);
+    if (st) return st;
+  }
+  if (sweeps == 0) return N3_OK;
+  WalkPlan plan;
+  st = walk_plan(chains->n, chains->replicas, &plan);
+  if (st) return st;
+  hipStream_t s = as_stream(stream);
+  const int per_graph = sweeps < 32 ? sweeps : 32;
+  const int full = sweeps / per_graph, rest = sweeps % per_graph;
+  GraphKey key{*chains, steps, ladder, bad_bias, per_graph, seed, s, 0};
+  (void)hipGetDevice(&key.device);
+  hipGraphExec_t exec = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_graph_mutex);
+    for (const auto& e : g_graphs)
+      if (same_key(e.key, key)) exec = e.exec;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      st = capture_graph(chains, plan, per_graph, steps, ladder, bad_bias, seed, &graph, &exec);
+      if (st) return st;
+      g_graphs.push_back(GraphEntry{key, exec, graph});
+    }
+  }
+  for (int i = 0; i < full; ++i) {
+    st = hip_status(hipGraphLaunch(exec, s), "hipGraphLaunch");
+    if (st) return st;
+  }
+  for (int i = 0; i < rest && !st; ++i) {
+    st = enqueue_advance(chains, plan, steps, bad_bias, seed, s);
+    if (!st && ladder > 1) st = enqueue_exchange(chains, ladder, seed, s);
+  }
+  return st;
+}
+
+extern "C" void n3_temper_release(void) {
+  std::lock_guard<std::mutex> lock(g_graph_mutex);
+  for (auto& e : g_graphs) {
+    (void)hipGraphExecDestroy(e.exec);
+    (void)hipGraphDestroy(e.graph);
+  }
+  g_graphs.clear();
+}
+
+// Host-side threshold tables shared by the device kernels and any CPU replay.
+extern "C" int n3_metropolis_thresholds(const float* host_temperatures, int replicas, int k, uint32_t* host_out) {
+  if (!host_temperatures || !host_out || replicas < 1 || k < 2) return set_error(N3_EINVAL, "bad threshold arguments");
+  for (int r = 0; r < replicas; ++r) {
+    const double t = (double)host_temperatures[r];
+    if (!(t > 0.0) || !std::isfinite(t)) return set_error(N3_EINVAL, "temperatures must be positive and finite");
+    host_out[(long long)r * k] = 0xFFFFFFFFu;
+    for (int d = 1; d < k; ++d) {
+      const double x = std::ldexp(std::exp(-(double)d / t), 32);
+      host_out[(long long)r * k + d] = x >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)std::floor(x);
+    }
+  }
+  return N3_OK;
+}
+
+extern "C" int n3_exchange_thresholds(const float* host_temperatures, int replicas, int ladder, int k,
+                                      uint32_t* host_out, int8_t* host_sign) {
+  if (!host_temperatures || !host_out || !host_sign || replicas < 1 || k < 2 || ladder < 1)
+    return set_error(N3_EINVAL, "bad threshold arguments");
+  for (int a = 0; a < replicas; ++a) {
+    int b = a + 1;
+    if ((a % ladder) == ladder - 1) b = a;
+    const double c = 1.0 / (double)host_temperatures[a] - 1.0 / (double)host_temperatures[b];
+    host_sign[a] = c > 0 ? 1 : (c < 0 ? -1 : 0);
+    const double ac = std::fabs(c);
+    host_out[(long long)a * k] = 0xFFFFFFFFu;
+    for (int d = 1; d < k; ++d) {
+      const double x = std::ldexp(std::exp(-ac * (double)d), 32);
+      host_out[(long long)a * k + d] = x >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)std::floor(x);
+    }
+  }
+  return N3_OK;
+}

@@ -1,0 +1,89 @@
+// Wavefront-level building blocks shared by the move kernels: DPP
+// reductions, uniform reads of per-lane slot arrays, and the
+// build/query/clear cycle of a byte-packed direction histogram in LDS.
+#pragma once
+
+#include "no3_common.h"
+
+namespace no3 {
+
+// Sum over the 64 lanes with DPP (quad perms, row mirrors, row broadcasts);
+// the total is read from lane 63 and returned to every lane as a uniform.
+// Unsigned, so that packed 16-bit fields may use the top bit.
+__device__ inline uint32_t wave_sum_dpp(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Value of a per-lane register array at a uniform slot (slot = lane + 64 c).
+template <int CH>
+__device__ inline int slot_value(const int (&arr)[CH], int slot) {
+  const int lane = slot & (kWave - 1), chunk = slot >> 6;
+  int r = __builtin_amdgcn_readlane(arr[0], lane);
+#pragma unroll
+  for (int c = 1; c < CH; ++c) {
+    const int t = __builtin_amdgcn_readlane(arr[c], lane);
+    r = (chunk == c) ? t : r;
+  }
+  return r;
+}
+
+// Histogram of the directions from `anchor` (a key) to the points keys[c] of
+// the lanes with include[c]; afterwards cnt[c] is the number of included
+// points on the line (anchor, keys[c]) and id[c] their direction id.  Every
+// lane executes every instruction (masked lanes count into the trash word),
+// so the LDS reads of all chunks are in flight together.  Every keys[c] must
+// be a valid point key.  The histogram is left cleared.
+template <int CH>
+__device__ inline void direction_counts(const uint16_t* table, uint32_t* hist, int trash, int offset, int anchor,
+                                        const int (&keys)[CH], const bool (&include)[CH], int (&cnt)[CH],
+                                        int (&id)[CH]) {
+  uint32_t word[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) id[c] = table[keys[c] - anchor + offset];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    word[c] = include[c] ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
+    atomicAdd(&hist[word[c]], 1u << (((uint32_t)id[c] << 3) & 31u));
+  }
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)((hist[word[c]] >> (((uint32_t)id[c] << 3) & 31u)) & 0xFFu);
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
+  wave_sync();
+}
+
+// Same as above when the caller already knows the ids.
+template <int CH>
+__device__ inline void direction_counts_known(uint32_t* hist, int trash, const int (&id)[CH], const bool (&include)[CH],
+                                              int (&cnt)[CH]) {
+  uint32_t word[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    word[c] = include[c] ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
+    atomicAdd(&hist[word[c]], 1u << (((uint32_t)id[c] << 3) & 31u));
+  }
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)((hist[word[c]] >> (((uint32_t)id[c] << 3) & 31u)) & 0xFFu);
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
+  wave_sync();
+}
+
+__device__ inline void stage_table(const uint16_t* __restrict__ table_g, uint16_t* table, int entries) {
+  const uint2* src = reinterpret_cast<const uint2*>(table_g);
+  uint2* dst = reinterpret_cast<uint2*>(table);
+  const int words = (entries * 2 + 7) / 8;
+  for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace no3

@@ -1,0 +1,600 @@
+/*
+ * CPU oracle for the no-three-in-line hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this file; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as the checker (or as the timed CPU
+ * baseline), never as the computation being measured or shipped.
+ *
+ * Every routine is a plain restatement of the reference algorithm it names
+ * (file:line in the reference repository MilliganCo/no-three-in-line), kept in
+ * the reference's own loop order so that integer results are bit-identical:
+ *
+ *   n3o_score_and_badness        accelerated.py:14-37   (i<j<k determinant loop)
+ *   n3o_score_slope              verifier_b.py:8-33     (normalized-direction counting form)
+ *   n3o_evaluate_transpositions  accelerated.py:40-119
+ *   n3o_evaluate_two_switches    accelerated.py:122-195
+ *   n3o_walk (tempering replay)  tempering.py:48-110    (walk kernel step semantics)
+ *   n3o_exchange                 tempering.py:169-191   (adjacent-temperature swaps)
+ *
+ * Deliberate, documented differences from the reference (the same in the
+ * product and in this oracle, so parity is exact):
+ *   - random numbers come from Philox4x32-10 (Salmon et al., SC'11) keyed by
+ *     the seed, with counters (move, sweep, slot, tag), instead of xorshift64*
+ *     (gpu_device_search.py:18-25) and the host PCG64 (tempering.py:145,182);
+ *   - Metropolis / exchange acceptance compares a 32-bit Philox word against a
+ *     precomputed integer threshold floor(2^32 * exp(-x)) instead of
+ *     log(U) < -x in double (tempering.py:92-95,181-182): same acceptance law
+ *     to within 2^-32, and free of libm differences between host and device;
+ *   - states for the benchmark populations come from the two-permutation
+ *     generator below (seeded Fisher-Yates + collision repair by a swap with a
+ *     random other row), not state.random_state's rejection loop.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define N3O_SENTINEL64 ((int64_t)1 << 60)
+#define N3O_MAXM 512
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10                                                              */
+/* ------------------------------------------------------------------------ */
+
+void n3o_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+static inline uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+    return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+#define TAG_GEN 0x47454E53u  /* "GENS" */
+#define TAG_MOVE 0x4D4F5645u /* "MOVE" */
+#define TAG_BIAS 0x42494153u /* "BIAS" */
+#define TAG_EXCH 0x45584348u /* "EXCH" */
+
+/* ------------------------------------------------------------------------ */
+/* Seeded two-permutation state generator                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t key[2];
+    uint32_t ctr[4];
+    uint32_t buf[4];
+    int pos;
+} gen_stream;
+
+static uint32_t gen_draw(gen_stream* s)
+{
+    if (s->pos == 4) {
+        n3o_philox4x32_10(s->ctr, s->key, s->buf);
+        s->ctr[2] += 1u;
+        s->pos = 0;
+    }
+    return s->buf[s->pos++];
+}
+
+/* State `index` of the population for `seed`: s1, s2 are Fisher-Yates
+ * shuffles (j = floor(u32 * (i+1) / 2^32)); every row with s1[i] == s2[i] is
+ * repaired, in increasing i, by swapping s2[i] with s2[r] for a random row
+ * r != i.  Such a swap can never create a new collision (s2[r] != s2[i] ==
+ * s1[i], and row r receives s1[i] != s1[r]), so one pass suffices. */
+void n3o_generate_state(uint64_t seed, int64_t index, int n, int32_t* p, int32_t* q)
+{
+    gen_stream s;
+    s.key[0] = (uint32_t)seed;
+    s.key[1] = (uint32_t)(seed >> 32);
+    s.ctr[0] = (uint32_t)index;
+    s.ctr[1] = (uint32_t)((uint64_t)index >> 32);
+    s.ctr[2] = 0;
+    s.ctr[3] = TAG_GEN;
+    s.pos = 4;
+    for (int i = 0; i < n; ++i) {
+        p[i] = i;
+        q[i] = i;
+    }
+    for (int i = n - 1; i >= 1; --i) {
+        int j = (int)mulhi32(gen_draw(&s), (uint32_t)(i + 1));
+        int32_t t = p[i];
+        p[i] = p[j];
+        p[j] = t;
+    }
+    for (int i = n - 1; i >= 1; --i) {
+        int j = (int)mulhi32(gen_draw(&s), (uint32_t)(i + 1));
+        int32_t t = q[i];
+        q[i] = q[j];
+        q[j] = t;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (q[i] == p[i]) {
+            int r = (int)mulhi32(gen_draw(&s), (uint32_t)(n - 1));
+            if (r >= i) r += 1;
+            int32_t t = q[i];
+            q[i] = q[r];
+            q[r] = t;
+        }
+    }
+}
+
+void n3o_generate_states(uint64_t seed, int n, int64_t count, int64_t first_index, int32_t* p, int32_t* q)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < count; ++s) n3o_generate_state(seed, first_index + s, n, p + s * n, q + s * n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Full scores                                                                */
+/* ------------------------------------------------------------------------ */
+
+static void interleave(const int32_t* p, const int32_t* q, int n, int32_t* rows, int32_t* cols)
+{
+    for (int r = 0; r < n; ++r) {
+        rows[2 * r] = r;
+        cols[2 * r] = p[r];
+        rows[2 * r + 1] = r;
+        cols[2 * r + 1] = q[r];
+    }
+}
+
+/* accelerated.py:14-37.  The collinear test and the (i, j, k) order are the
+ * reference's; the k loop keeps per-k hit flags so that the compiler can
+ * vectorise it, and the i and j counters are credited once per (i, j). */
+int64_t n3o_score_and_badness(const int32_t* p, const int32_t* q, int n, int64_t* badness)
+{
+    const int m = 2 * n;
+    int32_t rows[N3O_MAXM], cols[N3O_MAXM], bad[N3O_MAXM];
+    interleave(p, q, n, rows, cols);
+    for (int i = 0; i < m; ++i) bad[i] = 0;
+    int64_t score = 0;
+    for (int i = 0; i < m - 2; ++i) {
+        const int32_t ri = rows[i], ci = cols[i];
+        for (int j = i + 1; j < m - 1; ++j) {
+            const int32_t dx1 = rows[j] - ri;
+            const int32_t dy1 = cols[j] - ci;
+            int32_t hits = 0;
+            for (int k = j + 1; k < m; ++k) {
+                const int32_t h = (dx1 * (cols[k] - ci) == dy1 * (rows[k] - ri));
+                bad[k] += h;
+                hits += h;
+            }
+            bad[i] += hits;
+            bad[j] += hits;
+            score += hits;
+        }
+    }
+    if (badness)
+        for (int i = 0; i < m; ++i) badness[i] = bad[i];
+    return score;
+}
+
+/* Population form of the above (the reference's benchmark_gpu CPU leg,
+ * gpu.py:106-109, is the same loop over states), parallel over states. */
+void n3o_score_population(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores,
+                          int32_t* badness, int threads)
+{
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t s = 0; s < count; ++s) {
+        int64_t bad[N3O_MAXM];
+        const int64_t e = n3o_score_and_badness(p + s * n, q + s * n, n, bad);
+        scores[s] = (int32_t)e;
+        if (badness)
+            for (int i = 0; i < 2 * n; ++i) badness[s * 2 * n + i] = (int32_t)bad[i];
+    }
+}
+
+static int64_t gcd64(int64_t a, int64_t b)
+{
+    if (a < 0) a = -a;
+    if (b < 0) b = -b;
+    while (b) {
+        int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+/* Counting form of verifier_b.find_collinear_triple (verifier_b.py:8-33):
+ * per anchor, every other point's primitive undirected direction is counted;
+ * the anchor lies in sum_d C(cnt_d, 2) collinear triples and every triple is
+ * seen from its three members.  Returns -1 if two points coincide (the
+ * reference raises "duplicate point has no direction"). */
+int64_t n3o_score_slope(const int32_t* p, const int32_t* q, int n, int64_t* badness)
+{
+    const int m = 2 * n;
+    int32_t rows[N3O_MAXM], cols[N3O_MAXM];
+    interleave(p, q, n, rows, cols);
+    int32_t span = 1;
+    for (int i = 0; i < m; ++i) {
+        int32_t a = cols[i] < 0 ? -cols[i] : cols[i];
+        if (a + 1 > span) span = a + 1;
+    }
+    if (n > span) span = n;
+    const int32_t w = 2 * span + 1;
+    int32_t* counts = (int32_t*)calloc((size_t)w * (size_t)w, sizeof(int32_t));
+    int64_t total = 0;
+    for (int a = 0; a < m; ++a) {
+        int64_t pairs = 0;
+        for (int b = 0; b < m; ++b) {
+            if (b == a) continue;
+            int64_t dx = rows[b] - rows[a], dy = cols[b] - cols[a];
+            int64_t g = gcd64(dx, dy);
+            if (g == 0) {
+                free(counts);
+                return -1;
+            }
+            dx /= g;
+            dy /= g;
+            if (dx < 0 || (dx == 0 && dy < 0)) {
+                dx = -dx;
+                dy = -dy;
+            }
+            int32_t* c = &counts[(dx + span) * w + (dy + span)];
+            pairs += *c;
+            *c += 1;
+        }
+        for (int b = 0; b < m; ++b) {
+            if (b == a) continue;
+            int64_t dx = rows[b] - rows[a], dy = cols[b] - cols[a];
+            int64_t g = gcd64(dx, dy);
+            dx /= g;
+            dy /= g;
+            if (dx < 0 || (dx == 0 && dy < 0)) {
+                dx = -dx;
+                dy = -dy;
+            }
+            counts[(dx + span) * w + (dy + span)] = 0;
+        }
+        if (badness) badness[a] = pairs;
+        total += pairs;
+    }
+    free(counts);
+    return total / 3;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Incremental move scores                                                    */
+/* ------------------------------------------------------------------------ */
+
+static inline int det0(int32_t ar, int32_t ac, int32_t br, int32_t bc, int32_t cr, int32_t cc)
+{
+    return (br - ar) * (cc - ac) == (bc - ac) * (cr - ar);
+}
+
+/* new_affected of accelerated.py:161-193 for slots u, v moving to
+ * (rows[u], cols[v]) and (rows[v], cols[u]). */
+static int64_t new_affected_count(const int32_t* rows, const int32_t* cols, int m, int u, int v)
+{
+    const int32_t nur = rows[u], nuc = cols[v], nvr = rows[v], nvc = cols[u];
+    int64_t count = 0;
+    for (int i = 0; i < m - 1; ++i) {
+        if (i == u) continue;
+        int32_t ir, ic;
+        if (i == v) {
+            ir = nvr;
+            ic = nvc;
+        } else {
+            ir = rows[i];
+            ic = cols[i];
+        }
+        for (int j = i + 1; j < m; ++j) {
+            if (j == u) continue;
+            int32_t jr, jc;
+            if (j == v) {
+                jr = nvr;
+                jc = nvc;
+            } else {
+                jr = rows[j];
+                jc = cols[j];
+            }
+            if ((ir - nur) * (jc - nuc) == (ic - nuc) * (jr - nur)) count += 1;
+        }
+    }
+    for (int i = 0; i < m - 1; ++i) {
+        if (i == u || i == v) continue;
+        for (int j = i + 1; j < m; ++j) {
+            if (j == u || j == v) continue;
+            if ((rows[i] - nvr) * (cols[j] - nvc) == (cols[i] - nvc) * (rows[j] - nvr)) count += 1;
+        }
+    }
+    return count;
+}
+
+static int64_t joint_old_count(const int32_t* rows, const int32_t* cols, int m, int u, int v)
+{
+    int64_t joint = 0;
+    for (int o = 0; o < m; ++o) {
+        if (o == u || o == v) continue;
+        if ((rows[v] - rows[u]) * (cols[o] - cols[u]) == (cols[v] - cols[u]) * (rows[o] - rows[u])) joint += 1;
+    }
+    return joint;
+}
+
+/* accelerated.py:40-119 */
+void n3o_evaluate_transpositions(const int32_t* p, const int32_t* q, int n, int layer, int selected_row,
+                                 int64_t current_score, const int64_t* badness, int64_t* out)
+{
+    const int m = 2 * n;
+    int32_t rows[N3O_MAXM], cols[N3O_MAXM];
+    interleave(p, q, n, rows, cols);
+    const int u = 2 * selected_row + layer;
+    for (int partner = 0; partner < n; ++partner) {
+        out[partner] = N3O_SENTINEL64;
+        if (partner == selected_row) continue;
+        if (layer == 0) {
+            if (p[partner] == q[selected_row] || p[selected_row] == q[partner]) continue;
+        } else {
+            if (q[partner] == p[selected_row] || q[selected_row] == p[partner]) continue;
+        }
+        const int v = 2 * partner + layer;
+        const int64_t old_affected = badness[u] + badness[v] - joint_old_count(rows, cols, m, u, v);
+        out[partner] = current_score - old_affected + new_affected_count(rows, cols, m, u, v);
+    }
+}
+
+/* accelerated.py:122-195 */
+void n3o_evaluate_two_switches(const int32_t* p, const int32_t* q, int n, int selected_index, int64_t current_score,
+                               const int64_t* badness, int64_t* out)
+{
+    const int m = 2 * n;
+    int32_t rows[N3O_MAXM], cols[N3O_MAXM];
+    interleave(p, q, n, rows, cols);
+    const int u = selected_index;
+    const int selected_row = rows[u];
+    const int selected_other = 2 * selected_row + (1 - (u & 1));
+    for (int v = 0; v < m; ++v) {
+        out[v] = N3O_SENTINEL64;
+        if (v == u || rows[v] == selected_row) continue;
+        const int partner_other = 2 * rows[v] + (1 - (v & 1));
+        if (cols[v] == cols[selected_other] || cols[u] == cols[partner_other]) continue;
+        const int64_t old_affected = badness[u] + badness[v] - joint_old_count(rows, cols, m, u, v);
+        out[v] = current_score - old_affected + new_affected_count(rows, cols, m, u, v);
+    }
+}
+
+/* The full m x m 2-switch table of a population (analysis.py:57-74 evaluates
+ * the same function for every anchor): table[s][u][v] =
+ * evaluate_two_switches(state s, u)[v] with the int32 sentinel 1<<30 of
+ * gpu_delta.py:30.  Scores and badness are recomputed per state. */
+void n3o_delta_table(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* table)
+{
+    const int m = 2 * n;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t s = 0; s < count; ++s) {
+        int64_t bad[N3O_MAXM], out[N3O_MAXM];
+        const int64_t e = n3o_score_and_badness(p + s * n, q + s * n, n, bad);
+        for (int u = 0; u < m; ++u) {
+            n3o_evaluate_two_switches(p + s * n, q + s * n, n, u, e, bad, out);
+            for (int v = 0; v < m; ++v)
+                table[(s * m + u) * m + v] = out[v] >= N3O_SENTINEL64 ? (1 << 30) : (int32_t)out[v];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Replica-exchange tempering replay                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Acceptance thresholds: thr[r][d] = floor(2^32 * exp(-d / T_r)) for d >= 1
+ * (clamped to 2^32-1), thr[r][0] unused.  A move with integer energy change
+ * d > 0 is accepted iff its Philox word is < thr[r][min(d, k-1)]. */
+void n3o_metropolis_thresholds(const float* temperatures, int replicas, int k, uint32_t* thr)
+{
+    for (int r = 0; r < replicas; ++r) {
+        const double t = (double)temperatures[r];
+        thr[(int64_t)r * k] = 0xFFFFFFFFu;
+        for (int d = 1; d < k; ++d) {
+            double x = ldexp(exp(-(double)d / t), 32);
+            thr[(int64_t)r * k + d] = x >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)floor(x);
+        }
+    }
+}
+
+/* Exchange thresholds for the pair (a, a+1) inside each ladder:
+ * c = 1/T_a - 1/T_b (double from float32 temperatures), x = c * (E_a - E_b);
+ * accept iff x >= 0 or word < thr[a][min(|E_a - E_b|, k-1)] with
+ * thr[a][d] = floor(2^32 * exp(-|c| * d)).  The sign of c is stored in
+ * sign[a] (+1 / -1 / 0). */
+void n3o_exchange_thresholds(const float* temperatures, int replicas, int ladder, int k, uint32_t* thr, int8_t* sign)
+{
+    for (int a = 0; a < replicas; ++a) {
+        int b = a + 1;
+        if ((a % ladder) == ladder - 1) b = a;
+        const double c = 1.0 / (double)temperatures[a] - 1.0 / (double)temperatures[b];
+        sign[a] = c > 0 ? 1 : (c < 0 ? -1 : 0);
+        const double ac = fabs(c);
+        thr[(int64_t)a * k] = 0xFFFFFFFFu;
+        for (int d = 1; d < k; ++d) {
+            double x = ldexp(exp(-ac * (double)d), 32);
+            thr[(int64_t)a * k + d] = x >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)floor(x);
+        }
+    }
+}
+
+/* One replica (slot) running `steps` moves of the walk kernel
+ * (tempering.py:48-110) with umbrella strength 0.  Arrays are the slot's own
+ * rows of the caller's arrays.  Returns the number of valid proposals. */
+static int64_t walk_slot(int n, int steps, int bad_bias, uint64_t seed, uint32_t sweep, uint32_t slot,
+                         int32_t* col, int32_t* bad, int32_t* score_io, int32_t* best_col, int32_t* best_score,
+                         int32_t* accepted, const int32_t* reference, int32_t* removed_io, const uint32_t* thr, int k)
+{
+    const int m = 2 * n;
+    int32_t score = *score_io, removed = *removed_io;
+    int32_t db[N3O_MAXM];
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int64_t valid_count = 0;
+    for (int step = 0; step < steps; ++step) {
+        uint32_t ctr[4] = {(uint32_t)step, sweep, slot, TAG_MOVE}, r[4];
+        n3o_philox4x32_10(ctr, key, r);
+        int u = (int)mulhi32(r[0], (uint32_t)m);
+        if (bad_bias > 0 && score > 0 && (int)mulhi32(r[3], 100u) < bad_bias) {
+            uint32_t cb[4] = {(uint32_t)step, sweep, slot, TAG_BIAS}, rb[4];
+            n3o_philox4x32_10(cb, key, rb);
+            int64_t target = (int64_t)mulhi32(rb[0], (uint32_t)(3 * score));
+            u = 0;
+            while (u < m - 1 && target >= bad[u]) {
+                target -= bad[u];
+                ++u;
+            }
+        }
+        const int v = (int)mulhi32(r[1], (uint32_t)m);
+        const int32_t cu = col[u], cv = col[v];
+        const int valid = score > 0 && (u >> 1) != (v >> 1) && cu != cv && col[u ^ 1] != cv && col[v ^ 1] != cu;
+        const int ur = (u >> 1) * 2, vr = (v >> 1) * 2;
+        const int dd = (cu == reference[ur] || cu == reference[ur + 1]) + (cv == reference[vr] || cv == reference[vr + 1]) -
+                       (cv == reference[ur] || cv == reference[ur + 1]) - (cu == reference[vr] || cu == reference[vr + 1]);
+        if (!valid) continue;
+        valid_count += 1;
+        for (int i = 0; i < m; ++i) db[i] = 0;
+        int32_t delta = 0;
+        const int ru = u >> 1, rv = v >> 1;
+        /* tempering.py:71-82: triples with exactly one moved point */
+        for (int i = 0; i < m - 1; ++i) {
+            if (i == u || i == v) continue;
+            const int ri = i >> 1, ci = col[i];
+            for (int j = i + 1; j < m; ++j) {
+                if (j == u || j == v) continue;
+                const int rj = j >> 1, cj = col[j];
+                int d = det0(ru, cv, ri, ci, rj, cj) - det0(ru, cu, ri, ci, rj, cj);
+                if (d) {
+                    delta += d;
+                    db[u] += d;
+                    db[i] += d;
+                    db[j] += d;
+                }
+                d = det0(rv, cu, ri, ci, rj, cj) - det0(rv, cv, ri, ci, rj, cj);
+                if (d) {
+                    delta += d;
+                    db[v] += d;
+                    db[i] += d;
+                    db[j] += d;
+                }
+            }
+        }
+        /* tempering.py:84-88: triples with both moved points */
+        for (int kk = 0; kk < m; ++kk) {
+            if (kk == u || kk == v) continue;
+            const int d = det0(ru, cv, rv, cu, kk >> 1, col[kk]) - det0(ru, cu, rv, cv, kk >> 1, col[kk]);
+            if (d) {
+                delta += d;
+                db[u] += d;
+                db[v] += d;
+                db[kk] += d;
+            }
+        }
+        int accept = delta <= 0;
+        if (!accept) {
+            const int idx = delta < k ? delta : k - 1;
+            accept = r[2] < thr[idx];
+        }
+        if (accept) {
+            col[u] = cv;
+            col[v] = cu;
+            score += delta;
+            removed += dd;
+            *accepted += 1;
+            for (int i = 0; i < m; ++i) bad[i] += db[i];
+            if (score < *best_score) {
+                *best_score = score;
+                for (int i = 0; i < m; ++i) best_col[i] = col[i];
+            }
+        }
+    }
+    *score_io = score;
+    *removed_io = removed;
+    return valid_count;
+}
+
+/* TemperingChains.advance (tempering.py:158-167) for every slot. */
+int64_t n3o_advance(int n, int replicas, int steps, int bad_bias, uint64_t seed, uint32_t sweep, int32_t* columns,
+                    int32_t* badness, int32_t* scores, int32_t* best_columns, int32_t* best_scores, int32_t* accepted,
+                    const int32_t* reference, int32_t* removed, const uint32_t* thr, int k, int threads)
+{
+    const int m = 2 * n;
+    int64_t valid = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : valid)
+    for (int r = 0; r < replicas; ++r) {
+        valid += walk_slot(n, steps, bad_bias, seed, sweep, (uint32_t)r, columns + (int64_t)r * m,
+                           badness + (int64_t)r * m, scores + r, best_columns + (int64_t)r * m, best_scores + r,
+                           accepted + r, reference, removed + r, thr + (int64_t)r * k, k);
+    }
+    return valid;
+}
+
+/* TemperingChains.exchange (tempering.py:169-191): per ladder, pairs
+ * (a, a+1) from start + round%2 in steps of 2; an accepted pair swaps the
+ * slots' columns, badness, scores and removed counters.  Returns the number
+ * of accepted swaps. */
+int n3o_exchange(int n, int replicas, int ladder, uint64_t seed, uint32_t round, int32_t* columns, int32_t* badness,
+                 int32_t* scores, int32_t* removed, const uint32_t* thr, const int8_t* sign, int k)
+{
+    const int m = 2 * n;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int exchanged = 0;
+    for (int start = 0; start < replicas; start += ladder) {
+        for (int a = start + (int)(round % 2u); a < start + ladder - 1; a += 2) {
+            const int b = a + 1;
+            const int64_t diff = (int64_t)scores[a] - (int64_t)scores[b];
+            int accept;
+            if (sign[a] == 0 || (sign[a] > 0 && diff >= 0) || (sign[a] < 0 && diff <= 0)) {
+                accept = 1;
+            } else {
+                uint32_t ctr[4] = {(uint32_t)a, round, (uint32_t)(start / ladder), TAG_EXCH}, r[4];
+                n3o_philox4x32_10(ctr, key, r);
+                int64_t d = diff < 0 ? -diff : diff;
+                if (d > k - 1) d = k - 1;
+                accept = r[0] < thr[(int64_t)a * k + d];
+            }
+            if (accept) {
+                for (int i = 0; i < m; ++i) {
+                    int32_t t = columns[(int64_t)a * m + i];
+                    columns[(int64_t)a * m + i] = columns[(int64_t)b * m + i];
+                    columns[(int64_t)b * m + i] = t;
+                    t = badness[(int64_t)a * m + i];
+                    badness[(int64_t)a * m + i] = badness[(int64_t)b * m + i];
+                    badness[(int64_t)b * m + i] = t;
+                }
+                int32_t t = scores[a];
+                scores[a] = scores[b];
+                scores[b] = t;
+                t = removed[a];
+                removed[a] = removed[b];
+                removed[b] = t;
+                exchanged += 1;
+            }
+        }
+    }
+    return exchanged;
+}

@@ -1,0 +1,206 @@
+"""ctypes front-end of the CPU oracle (``oracle.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py``, always as the checker or as the
+timed CPU baseline, never by the product package.
+
+Each function mirrors the reference routine cited in ``oracle.c``; the array
+conventions follow the reference (``p``/``q`` int32 permutations of length n,
+badness int64 on the CPU side as in accelerated.py:25).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+SENTINEL64 = 1 << 60
+SENTINEL32 = 1 << 30
+
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> Path:
+    """Compile liboracle.so with the Makefile next to this file."""
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "oracle.c").stat().st_mtime:
+        subprocess.run(["make", "-B" if force else "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.n3o_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.n3o_generate_states.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _i32p, _i32p]
+        L.n3o_score_and_badness.argtypes = [_i32p, _i32p, ctypes.c_int, _i64p]
+        L.n3o_score_and_badness.restype = ctypes.c_int64
+        L.n3o_score_slope.argtypes = [_i32p, _i32p, ctypes.c_int, _i64p]
+        L.n3o_score_slope.restype = ctypes.c_int64
+        L.n3o_score_population.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_int64, _i32p, ctypes.c_void_p, ctypes.c_int]
+        L.n3o_evaluate_transpositions.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _i64p, _i64p]
+        L.n3o_evaluate_two_switches.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _i64p, _i64p]
+        L.n3o_delta_table.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_int64, _i32p]
+        L.n3o_metropolis_thresholds.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _u32p]
+        L.n3o_exchange_thresholds.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, _i8p]
+        L.n3o_advance.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+            _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _u32p, ctypes.c_int, ctypes.c_int,
+        ]
+        L.n3o_advance.restype = ctypes.c_int64
+        L.n3o_exchange.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+            _i32p, _i32p, _i32p, _i32p, _u32p, _i8p, ctypes.c_int,
+        ]
+        L.n3o_exchange.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def philox4x32_10(counter, key) -> np.ndarray:
+    out = np.zeros(4, dtype=np.uint32)
+    lib().n3o_philox4x32_10(np.asarray(counter, dtype=np.uint32), np.asarray(key, dtype=np.uint32), out)
+    return out
+
+
+def generate_states(seed: int, n: int, count: int, first_index: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    p = np.empty((count, n), dtype=np.int32)
+    q = np.empty((count, n), dtype=np.int32)
+    lib().n3o_generate_states(seed, n, count, first_index, p, q)
+    return p, q
+
+
+def score_and_badness(p, q) -> tuple[int, np.ndarray]:
+    """accelerated.py:14-37."""
+    p, q = _i32(p), _i32(q)
+    bad = np.zeros(2 * len(p), dtype=np.int64)
+    score = lib().n3o_score_and_badness(p, q, len(p), bad)
+    return int(score), bad
+
+
+def score_slope(p, q) -> tuple[int, np.ndarray]:
+    """Normalized-direction counting (verifier_b.py:8-33); raises on coincident points."""
+    p, q = _i32(p), _i32(q)
+    bad = np.zeros(2 * len(p), dtype=np.int64)
+    score = lib().n3o_score_slope(p, q, len(p), bad)
+    if score < 0:
+        raise ValueError("duplicate point has no direction")
+    return int(score), bad
+
+
+def score_population(p_batch, q_batch, *, badness: bool = True, threads: int = 0):
+    p, q = _i32(p_batch), _i32(q_batch)
+    count, n = p.shape
+    scores = np.empty(count, dtype=np.int32)
+    bad = np.empty((count, 2 * n), dtype=np.int32) if badness else None
+    lib().n3o_score_population(p, q, n, count, scores, None if bad is None else bad.ctypes.data, threads)
+    return scores, bad
+
+
+def evaluate_transpositions(p, q, layer: int, selected_row: int, current_score: int, badness) -> np.ndarray:
+    """accelerated.py:40-119 (int64, sentinel 1<<60)."""
+    p, q = _i32(p), _i32(q)
+    out = np.empty(len(p), dtype=np.int64)
+    lib().n3o_evaluate_transpositions(p, q, len(p), layer, selected_row, current_score,
+                                      np.ascontiguousarray(badness, dtype=np.int64), out)
+    return out
+
+
+def evaluate_two_switches(p, q, selected_index: int, current_score: int, badness) -> np.ndarray:
+    """accelerated.py:122-195 (int64, sentinel 1<<60)."""
+    p, q = _i32(p), _i32(q)
+    out = np.empty(2 * len(p), dtype=np.int64)
+    lib().n3o_evaluate_two_switches(p, q, len(p), selected_index, current_score,
+                                    np.ascontiguousarray(badness, dtype=np.int64), out)
+    return out
+
+
+def delta_table(p_batch, q_batch) -> np.ndarray:
+    """Full m x m 2-switch table per state, int32 with sentinel 1<<30."""
+    p, q = _i32(p_batch), _i32(q_batch)
+    count, n = p.shape
+    table = np.empty((count, 2 * n, 2 * n), dtype=np.int32)
+    lib().n3o_delta_table(p, q, n, count, table)
+    return table
+
+
+def metropolis_thresholds(temperatures, k: int) -> np.ndarray:
+    t = np.ascontiguousarray(temperatures, dtype=np.float32)
+    thr = np.empty((len(t), k), dtype=np.uint32)
+    lib().n3o_metropolis_thresholds(t, len(t), k, thr)
+    return thr
+
+
+def exchange_thresholds(temperatures, ladder: int, k: int) -> tuple[np.ndarray, np.ndarray]:
+    t = np.ascontiguousarray(temperatures, dtype=np.float32)
+    thr = np.empty((len(t), k), dtype=np.uint32)
+    sign = np.empty(len(t), dtype=np.int8)
+    lib().n3o_exchange_thresholds(t, len(t), ladder, k, thr, sign)
+    return thr, sign
+
+
+class TemperingReplay:
+    """CPU replay of the tempering state machine (tempering.py:119-191) on the
+    Philox stream; arrays are slot-ordered like TemperingChains'."""
+
+    def __init__(self, columns, temperatures, seed: int, *, reference=None, k_move: int, k_exchange: int):
+        self.columns = _i32(columns).copy()
+        self.replicas, m = self.columns.shape
+        self.n = m // 2
+        self.temperatures = np.ascontiguousarray(temperatures, dtype=np.float32)
+        self.seed = int(seed)
+        scores, bad = [], []
+        for row in self.columns:
+            s, b = score_and_badness(row[0::2], row[1::2])
+            scores.append(s)
+            bad.append(b)
+        self.scores = np.asarray(scores, dtype=np.int32)
+        self.badness = np.asarray(bad, dtype=np.int32)
+        self.best_columns = self.columns.copy()
+        self.best_scores = self.scores.copy()
+        self.accepted = np.zeros(self.replicas, dtype=np.int32)
+        self.reference = _i32(self.columns[0] if reference is None else reference).copy()
+        ref = set((i // 2, int(c)) for i, c in enumerate(self.reference))
+        self.removed_points = np.asarray(
+            [len(ref - set((i // 2, int(c)) for i, c in enumerate(row))) for row in self.columns], dtype=np.int32)
+        self.k_move = k_move
+        self.k_exchange = k_exchange
+        self.thr = metropolis_thresholds(self.temperatures, k_move)
+        self.sweep = 0
+        self.round = 0
+
+    def advance(self, steps: int, bad_bias: int = 0, threads: int = 0) -> int:
+        valid = lib().n3o_advance(
+            self.n, self.replicas, steps, bad_bias, self.seed, self.sweep, self.columns, self.badness, self.scores,
+            self.best_columns, self.best_scores, self.accepted, self.reference, self.removed_points, self.thr,
+            self.k_move, threads)
+        self.sweep += 1
+        return int(valid)
+
+    def exchange(self, ladder_size: int) -> int:
+        thr, sign = exchange_thresholds(self.temperatures, ladder_size, self.k_exchange)
+        count = lib().n3o_exchange(self.n, self.replicas, ladder_size, self.seed, self.round, self.columns,
+                                   self.badness, self.scores, self.removed_points, thr, sign, self.k_exchange)
+        self.round += 1
+        return int(count)
+
+
+def cpu_count() -> int:
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)

@@ -1,0 +1,96 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol declared in
+include/no3.h, and its host-only functions agree with the oracle."""
+import re
+from math import gcd
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "no3.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(n3_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_header(pkg):
+    lib = pkg._lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) <= set(pkg._lib.SIGNATURES)
+    assert lib.n3_version() >= 10000
+
+
+def test_direction_table(pkg):
+    import ctypes
+
+    lib = pkg._lib.lib()
+    for n in (2, 5, 12, 75):
+        w = 2 * n - 1
+        table = np.empty(w * w, np.uint16)
+        d = ctypes.c_int()
+        hw = ctypes.c_int()
+        assert lib.n3_dirtable(n, table.ctypes.data, ctypes.byref(d), ctypes.byref(hw)) == 0
+        table = table.reshape(w, w)
+        ids = {}
+        for dr in range(-(n - 1), n):
+            for dc in range(-(n - 1), n):
+                v = int(table[dr + n - 1, dc + n - 1])
+                if dr == 0 and dc == 0:
+                    assert v == d.value
+                    continue
+                g = gcd(abs(dr), abs(dc))
+                a, b = dr // g, dc // g
+                if a < 0 or (a == 0 and b < 0):
+                    a, b = -a, -b
+                assert ids.setdefault((a, b), v) == v
+        assert len(set(ids.values())) == len(ids) == d.value
+        assert hw.value * 4 >= d.value + 1 + 4
+    # the count of canonical directions at n=75 quoted for the reference grid
+    assert lib.n3_dirtable(75, None, ctypes.byref(d), None) == 0 and d.value == 6784
+    assert lib.n3_dirtable(101, None, None, None) == pkg._lib.N3_EINVAL
+
+
+def test_thresholds_match_oracle(pkg, oracle):
+    from no_three_in_line_amd import tempering
+
+    temps = np.tile(np.geomspace(0.05, 2.0, 64), 2).astype(np.float32)
+    k = tempering.move_table_size(temps)
+    assert np.array_equal(tempering.metropolis_thresholds(temps, k), oracle.metropolis_thresholds(temps, k))
+    kx = tempering.exchange_table_size(temps, 64)
+    thr, sign = tempering.exchange_thresholds(temps, 64, kx)
+    othr, osign = oracle.exchange_thresholds(temps, 64, kx)
+    assert np.array_equal(thr, othr) and np.array_equal(sign, osign)
+    assert np.all(tempering.metropolis_thresholds(temps, k)[:, -1] == 0)
+
+
+def test_host_validation_errors(pkg):
+    import pytest
+
+    with pytest.raises(ValueError):
+        pkg.TemperingChains(np.zeros((4, 7), np.int32), np.ones(4, np.float32), 0)
+    with pytest.raises(ValueError):
+        pkg.TemperingChains(np.zeros((4, 8), np.int32), np.ones(3, np.float32), 0)
+    cols = np.tile(np.array([0, 1, 1, 2, 2, 3, 3, 0], np.int32), (2, 1))
+    bad = cols.copy()
+    bad[0, 1] = 0  # row 0 uses column 0 twice
+    with pytest.raises(ValueError):
+        pkg.TemperingChains(bad, np.ones(2, np.float32), 0)
+    with pytest.raises(ValueError):
+        pkg.score_population_gpu(np.zeros((3, 4)), np.zeros((3, 5)))
+
+
+def test_points_to_permutations(pkg):
+    from random import Random
+
+    rng = Random(123)
+    state = pkg.random_state(75, rng)
+    state.validate()
+    again = pkg.points_to_permutations(state.points(), 75)
+    assert again.points() == state.points()
+    # matching colourings keep the point set (tests/test_state.py:34-38)
+    s = pkg.PairPermutations((0, 1, 2, 3, 4, 5), (1, 0, 3, 2, 5, 4))
+    assert pkg.points_to_permutations(s.points(), 6).points() == s.points()

@@ -1,0 +1,266 @@
+"""Pins the CPU oracle: published known answers, the reference's fixture and
+test states, and independent formulations on small inputs."""
+import json
+from collections import Counter
+from itertools import combinations
+from math import comb, gcd
+from pathlib import Path
+from random import Random
+
+import numpy as np
+import pytest
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+# -- independent formulations (brute force on tiny inputs) --------------------
+
+def canonical_line(a, b):
+    """Primitive (A, B, C) of the line through a and b with A > 0 or A == 0 < B
+    (the form of geometry.canonical_line, geometry.py:14-28)."""
+    A = b[1] - a[1]
+    B = a[0] - b[0]
+    C = -(A * a[0] + B * a[1])
+    g = gcd(gcd(abs(A), abs(B)), abs(C))
+    A, B, C = A // g, B // g, C // g
+    if A < 0 or (A == 0 and B < 0):
+        A, B, C = -A, -B, -C
+    return A, B, C
+
+
+def line_count(points):
+    """E = sum over lines of C(k, 3) and badness = sum of C(k-1, 2) over the
+    lines through each point (conflicts.analyze_points, conflicts.py:36-72)."""
+    members = {}
+    for i, j in combinations(range(len(points)), 2):
+        key = canonical_line(points[i], points[j])
+        members.setdefault(key, set()).update((i, j))
+    badness = [0] * len(points)
+    triples = 0
+    for idx in members.values():
+        k = len(idx)
+        if k >= 3:
+            triples += comb(k, 3)
+            for i in idx:
+                badness[i] += comb(k - 1, 2)
+    return triples, badness
+
+
+def slots(p, q):
+    pts = []
+    for r in range(len(p)):
+        pts.append((r, int(p[r])))
+        pts.append((r, int(q[r])))
+    return pts
+
+
+# -- Philox ---------------------------------------------------------------------
+
+def test_philox_known_answers(oracle):
+    """Random123 kat_vectors for philox4x32_10."""
+    kats = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, expected in kats:
+        assert list(oracle.philox4x32_10(ctr, key)) == expected
+
+
+# -- scoring --------------------------------------------------------------------
+
+def test_determinant_known_answers(oracle):
+    """tests/test_geometry.py:5-8 of the reference, as 3-point states' rows:
+    the oracle's collinearity test on the same triples."""
+    def collinear3(a, b, c):
+        # one P/Q state cannot hold arbitrary triples; use the line counter on
+        # the triple, and the oracle on a state containing it where possible
+        return line_count([a, b, c])[0] == 1
+    assert collinear3((0, 0), (3, 2), (6, 4))
+    assert not collinear3((0, 0), (3, 2), (6, 5))
+    assert collinear3((2, 0), (2, 4), (2, 9))
+    assert collinear3((0, 7), (4, 7), (9, 7))
+    assert collinear3((0, 0), (2, 2), (5, 5))  # opposite rays: test_verifier_agreement.py:18-21
+
+
+def test_near_solution_fixture(oracle):
+    """results/best-near-solution-E10.json: E=10 (README.md:7), and the slope
+    form, the determinant form and the line form agree."""
+    data = json.loads((GOLDEN / "best-near-solution-E10.json").read_text())
+    n = data["n"]
+    adj = [[] for _ in range(n)]
+    for r, c in data["points"]:
+        adj[r].append(c)
+    p = np.array([min(a) for a in adj], np.int32)
+    q = np.array([max(a) for a in adj], np.int32)
+    e, bad = oracle.score_and_badness(p, q)
+    e2, bad2 = oracle.score_slope(p, q)
+    e3, bad3 = line_count(slots(p, q))
+    assert e == e2 == e3 == 10
+    assert list(bad) == list(bad2) == bad3
+    assert bad.sum() == 30 and bad.max() == 3 and (bad > 0).sum() == 24
+
+
+def test_scores_match_line_form_small(oracle):
+    rng = np.random.default_rng(1)
+    for n in range(2, 13):
+        for _ in range(6):
+            p = rng.permutation(n).astype(np.int32)
+            q = rng.permutation(n).astype(np.int32)
+            e, bad = oracle.score_and_badness(p, q)
+            t, b = line_count(slots(p, q)) if not np.any(p == q) else (None, None)
+            if t is not None:
+                assert e == t and list(bad) == b
+                e2, bad2 = oracle.score_slope(p, q)
+                assert e2 == e and np.array_equal(bad2, bad)
+
+
+def test_coincident_points_counted_like_determinant(oracle):
+    """With P[r] == Q[r] every triple holding both copies is collinear; the
+    slope form has no direction for the pair and refuses."""
+    p = np.array([0, 1, 2, 3], np.int32)
+    q = np.array([0, 2, 3, 1], np.int32)
+    e, bad = oracle.score_and_badness(p, q)
+    pts = slots(p, q)
+    brute = sum(1 for a, b, c in combinations(pts, 3)
+                if (b[0] - a[0]) * (c[1] - a[1]) == (b[1] - a[1]) * (c[0] - a[0]))
+    assert e == brute
+    with pytest.raises(ValueError):
+        oracle.score_slope(p, q)
+
+
+def test_population_scores_random_n75(oracle):
+    p, q = oracle.generate_states(0, 75, 24)
+    s, b = oracle.score_population(p, q)
+    for i in range(4):
+        e2, bad2 = oracle.score_slope(p[i], q[i])
+        assert s[i] == e2 and np.array_equal(b[i], bad2)
+    assert 350 < s.mean() < 550  # E of random n=75 states is about 445
+
+
+# -- generator ------------------------------------------------------------------
+
+def test_generator_invariants(oracle):
+    for n in (2, 3, 32, 75, 100):
+        p, q = oracle.generate_states(0, n, 200)
+        assert np.all(np.sort(p, axis=1) == np.arange(n))
+        assert np.all(np.sort(q, axis=1) == np.arange(n))
+        assert np.all(p != q)
+    a, _ = oracle.generate_states(0, 32, 10, first_index=5)
+    b, _ = oracle.generate_states(0, 32, 15)
+    assert np.array_equal(a, b[5:])
+
+
+# -- deltas: the reference's own test states -----------------------------------
+
+def _random_state(n, rng):
+    """state.random_state (state.py:88-95) on stdlib Random."""
+    p = list(range(n))
+    q = list(range(n))
+    rng.shuffle(p)
+    while True:
+        rng.shuffle(q)
+        if all(p[r] != q[r] for r in range(n)):
+            return p, q
+
+
+def test_transpositions_match_full_recompute(oracle):
+    """tests/test_delta.py:10-30 of the reference (Random(9817))."""
+    rng = Random(9817)
+    comparisons = 0
+    for n in range(4, 13):
+        for _ in range(8):
+            P, Q = _random_state(n, rng)
+            p = np.asarray(P, np.int32)
+            q = np.asarray(Q, np.int32)
+            score, badness = oracle.score_and_badness(p, q)
+            assert score == line_count(slots(p, q))[0]
+            for _ in range(4):
+                layer = rng.randrange(2)
+                row = rng.randrange(n)
+                cand = oracle.evaluate_transpositions(p, q, layer, row, score, badness)
+                for partner in range(n):
+                    if cand[partner] >= (1 << 59):
+                        continue
+                    moved = [list(P), list(Q)]
+                    moved[layer][row], moved[layer][partner] = moved[layer][partner], moved[layer][row]
+                    assert moved[0][row] != moved[1][row] and moved[0][partner] != moved[1][partner]
+                    assert cand[partner] == line_count(slots(moved[0], moved[1]))[0]
+                    comparisons += 1
+    assert comparisons >= 1000
+
+
+def test_two_switches_match_full_recompute(oracle):
+    """tests/test_delta.py:33-60 of the reference (Random(443))."""
+    rng = Random(443)
+    comparisons = 0
+    for n in range(4, 11):
+        for _ in range(5):
+            P, Q = _random_state(n, rng)
+            p = np.asarray(P, np.int32)
+            q = np.asarray(Q, np.int32)
+            score, badness = oracle.score_and_badness(p, q)
+            selected = rng.randrange(2 * n)
+            cand = oracle.evaluate_two_switches(p, q, selected, score, badness)
+            pts = slots(p, q)
+            for partner in range(2 * n):
+                if cand[partner] >= (1 << 59):
+                    continue
+                moved = list(pts)
+                (ra, ca), (rb, cb) = pts[selected], pts[partner]
+                moved[selected] = (ra, cb)
+                moved[partner] = (rb, ca)
+                assert cand[partner] == line_count(moved)[0]
+                comparisons += 1
+    assert comparisons >= 200
+
+
+def test_delta_table_rows(oracle):
+    p, q = oracle.generate_states(0, 8, 3)
+    table = oracle.delta_table(p, q)
+    for s in range(3):
+        score, bad = oracle.score_and_badness(p[s], q[s])
+        for u in range(16):
+            ref = oracle.evaluate_two_switches(p[s], q[s], u, score, bad)
+            exp = np.where(ref >= (1 << 59), 1 << 30, ref).astype(np.int32)
+            assert np.array_equal(table[s, u], exp)
+
+
+# -- tempering replay -----------------------------------------------------------
+
+def test_replay_invariants(oracle):
+    """The walk's incremental score and badness equal a full recount after
+    every sweep, and exchanges permute states between slots."""
+    n, replicas, ladder = 9, 8, 4
+    p, q = oracle.generate_states(3, n, replicas)
+    cols = np.stack([np.column_stack((p[i], q[i])).reshape(-1) for i in range(replicas)]).astype(np.int32)
+    temps = np.tile(np.geomspace(0.2, 3.0, ladder), 2).astype(np.float32)
+    r = oracle.TemperingReplay(cols, temps, 17, k_move=80, k_exchange=400)
+    for _ in range(30):
+        r.advance(18)
+        before = sorted(map(tuple, r.columns))
+        r.exchange(ladder)
+        assert sorted(map(tuple, r.columns)) == before
+        for i in range(replicas):
+            e, bad = oracle.score_and_badness(r.columns[i, 0::2], r.columns[i, 1::2])
+            assert e == r.scores[i] and np.array_equal(bad, r.badness[i])
+            c = Counter(r.columns[i])
+            assert all(v == 2 for v in c.values()) and len(c) == n
+    assert r.accepted.sum() > 0
+
+
+def test_metropolis_thresholds_law(oracle):
+    thr = oracle.metropolis_thresholds(np.array([0.5, 2.0], np.float32), 64)
+    for t, row in zip((0.5, 2.0), thr):
+        for d in (1, 2, 5, 20):
+            assert abs(row[d] / 2**32 - np.exp(-d / t)) < 2**-31
+        assert row[-1] == 0
+
+
+def test_exchange_thresholds_law(oracle):
+    temps = np.array([0.1, 0.2, 0.4, 0.4], np.float32)
+    thr, sign = oracle.exchange_thresholds(temps, 4, 32)
+    assert list(sign) == [1, 1, 0, 0]
+    c = 1 / np.float64(np.float32(0.1)) - 1 / np.float64(np.float32(0.2))
+    assert abs(thr[0, 3] / 2**32 - np.exp(-c * 3)) < 2**-31

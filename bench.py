@@ -113,6 +113,20 @@ def tempering_rate(pkg, torch, sweeps: int = 100) -> dict:
             "best_E": int(chains.best_scores.min().item())}
 
 
+def calibrate_copy() -> int:
+    """A plain device-to-device copy of 256 MiB, five times: profiled with
+    FETCH_SIZE / WRITE_SIZE it gives the counters' reading for known bytes."""
+    import torch
+
+    x = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 5}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,7 +135,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-temper", action="store_true", help="skip the tempering rate")
     ap.add_argument("--method", type=int, default=0, help="0 auto (direction histograms), 1 determinant")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine shards by whole "
               f"populations/ladders and runs no collectives", file=sys.stderr)

@@ -116,6 +116,7 @@ typedef struct n3_chains {
     int k_exchange;
     uint32_t* counters;      /* [4]: sweep, round, exchanges (low), exchanges (high) */
     int64_t* proposals;      /* [2]: valid proposals (total), accepted moves (total) */
+    int block_waves;         /* replicas (wavefronts) per workgroup; 0 = automatic */
 } n3_chains;
 
 /* `steps` proposals per replica (one sweep), TemperingChains.advance
@@ -137,6 +138,27 @@ int n3_temper_exchange(const n3_chains* chains, int ladder, uint64_t seed, void*
  * they change. */
 int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int ladder, int bad_bias, uint64_t seed,
                   void* stream);
+
+/* One population of a batched run: its chains, exchange ladder size (0 or 1
+ * = no exchange), proposals per replica per sweep, bad_bias and seed. */
+typedef struct n3_group {
+    n3_chains chains;
+    int ladder;
+    int steps;
+    int bad_bias;
+    uint64_t seed;
+} n3_group;
+
+#define N3_MAX_GROUPS 8
+
+/* Populations of different sizes (2 <= n <= 100 each) advanced together:
+ * every sweep is ONE walk launch over all groups' replicas (each group with
+ * its own LDS direction table and chains.block_waves replicas per
+ * workgroup), one exchange launch and one counter update, replayed `sweeps`
+ * times from a captured hipGraph.  Group g's trajectory is bit-identical to
+ * n3_temper_run on that group alone.  No reference counterpart (the
+ * reference caps n at 75 and runs one size per process, tempering.py:124). */
+int n3_temper_batched_run(const n3_group* groups, int group_count, int sweeps, void* stream);
 
 /* Drops cached graphs (all of them). */
 void n3_temper_release(void);

@@ -51,7 +51,23 @@ class Chains(ctypes.Structure):
         ("k_exchange", ctypes.c_int),
         ("counters", ctypes.c_void_p),
         ("proposals", ctypes.c_void_p),
+        ("block_waves", ctypes.c_int),
     ]
+
+
+class Group(ctypes.Structure):
+    """n3_group (include/no3.h)."""
+
+    _fields_ = [
+        ("chains", Chains),
+        ("ladder", ctypes.c_int),
+        ("steps", ctypes.c_int),
+        ("bad_bias", ctypes.c_int),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+MAX_GROUPS = 8
 
 
 _vp = ctypes.c_void_p
@@ -75,6 +91,7 @@ SIGNATURES = {
     "n3_temper_advance": (_i, [ctypes.POINTER(Chains), _i, _i, _u64, _vp]),
     "n3_temper_exchange": (_i, [ctypes.POINTER(Chains), _i, _u64, _vp]),
     "n3_temper_run": (_i, [ctypes.POINTER(Chains), _i, _i, _i, _i, _u64, _vp]),
+    "n3_temper_batched_run": (_i, [ctypes.POINTER(Group), _i, _i, _vp]),
     "n3_temper_release": (None, []),
     "n3_metropolis_thresholds": (_i, [_vp, _i, _i, _vp]),
     "n3_exchange_thresholds": (_i, [_vp, _i, _i, _i, _vp, _vp]),

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
 #pragma unroll
     for (int c = 0; c < CH; ++c)
       if (lane + kWave * c >= m) key[c] = key0;
-    const int trash = trash_word(info);
+    const int trash = trash_word(info, lane);
     const int ru = u >> 1;
     const int key_u = slot_value<CH>(key, u);
     const int cu = key_u - ru * width;

@@ -40,11 +40,14 @@ struct DirTableInfo {
   int entries;      // W * W
   int directions;   // D(n); the zero vector has id D(n)
   int hist_words;   // 32-bit words of one byte-packed histogram: the D+1
-                    // byte counters, then a spare "trash" word that masked
-                    // lanes update instead of branching around the update
+                    // byte counters, then 64 spare "trash" words, one per
+                    // lane, that masked lanes update instead of branching
+                    // around the update (one word per lane: no conflicts)
 };
 
-__host__ __device__ inline int trash_word(const DirTableInfo& info) { return info.hist_words - 1; }
+__host__ __device__ inline int trash_word(const DirTableInfo& info, int lane) {
+  return info.hist_words - kWave + lane;
+}
 
 __host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
   return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);

@@ -65,11 +65,12 @@ int build_dirtable(int n, std::vector<uint16_t>& table, DirTableInfo& info) {
   info.offset = span * w + span;
   info.entries = w * w;
   info.directions = next;
-  // byte counters, four per 32-bit word, plus the trash word, rounded up to
-  // 16 words (64 bytes); at least 4n words, the scratch the determinant
-  // fallback keeps there
-  int words = (next + 1 + 3) / 4 + 1;
+  // byte counters, four per 32-bit word (at least 4n words, the scratch the
+  // determinant fallback keeps there), then one trash word per lane, rounded
+  // up to 16 words (64 bytes)
+  int words = (next + 1 + 3) / 4;
   if (words < 4 * n) words = 4 * n;
+  words += kWave;
   info.hist_words = (words + 15) / 16 * 16;
   return N3_OK;
 }

@@ -204,12 +204,11 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
   const int m = 2 * n;
   const int width = info.width;
   for (long long s = (long long)blockIdx.x * waves + wave; s < count; s += (long long)gridDim.x * waves) {
-    int key[CH], acc[CH];
+    int key[CH];
     int outside = 0, coincide = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      acc[c] = 0;
       key[c] = 0;
       if (k < m) {
         const int col = (k & 1) ? q[s * n + (k >> 1)] : p[s * n + (k >> 1)];
@@ -256,44 +255,65 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
       continue;
     }
 
-    // lanes past the last slot carry a valid key (slot 0's) and count into
-    // the trash word, so that every lane runs every instruction
+    // lanes past the last slot carry a valid key (slot 0's); they sit out
+    // the LDS work of the last chunk under the exec mask
     const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
 #pragma unroll
     for (int c = 0; c < CH; ++c)
       if (lane + kWave * c >= m) key[c] = key0;
-    const int trash = trash_word(info);
+    // badness of anchor a = sum over its direction histogram of C(cnt, 2) =
+    // sum of the values the lanes' increments return (each point's rank
+    // among the earlier points of its direction); the anchor's own lane hits
+    // the zero-vector bin alone and adds 0
+    // One anchor: histogram build with returned ranks, then clear.  Only the
+    // last chunk can hold lanes past the last slot (m > 64 (CH-1)).  The
+    // shift (id << 3) is used unmasked: the shifts and bit-field extracts
+    // read its low five bits.
+    auto anchor_part = [&](int anchor_key) -> uint32_t {
+      const int base = info.offset - anchor_key;
+      constexpr int L = CH - 1;  // the only chunk that can hold idle lanes
+      const bool tail = lane + kWave * L < m;
+      uint32_t id[CH], old[CH];
+#pragma unroll
+      for (int c = 0; c < L; ++c) id[c] = table[key[c] + base];
+      id[L] = 0;
+      if (tail) id[L] = table[key[L] + base];
+#pragma unroll
+      for (int c = 0; c < L; ++c) old[c] = atomicAdd(&hist[id[c] >> 2], 1u << (id[c] << 3));
+      old[L] = 0;
+      if (tail) old[L] = atomicAdd(&hist[id[L] >> 2], 1u << (id[L] << 3));
+      wave_sync();
+#pragma unroll
+      for (int c = 0; c < L; ++c) hist[id[c] >> 2] = 0u;
+      if (tail) hist[id[L] >> 2] = 0u;
+      wave_sync();
+      uint32_t part = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) part += __builtin_amdgcn_ubfe(old[c], id[c] << 3, 8);
+      return part;
+    };
+    int bad[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) bad[c] = 0;
 #pragma unroll
     for (int ca = 0; ca < CH; ++ca) {
       const int limit = min(kWave, m - kWave * ca);
-      for (int l = 0; l < limit; ++l) {
-        const int base = info.offset - __builtin_amdgcn_readlane(key[ca], l);
-        uint32_t id[CH], word[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) id[c] = table[key[c] + base];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          word[c] = (lane + kWave * c < m) ? (id[c] >> 2) : (uint32_t)trash;
-          atomicAdd(&hist[word[c]], 1u << ((id[c] << 3) & 31u));
-        }
-        wave_sync();
-#pragma unroll
-        for (int c = 0; c < CH; ++c) acc[c] += (int)((hist[word[c]] >> ((id[c] << 3) & 31u)) & 0xFFu);
-        wave_sync();
-#pragma unroll
-        for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
-        wave_sync();
+      for (int l = 0; l < limit; l += 2) {
+        // two anchors per reduction: a lane's part is at most 3 x 148 and an
+        // anchor's total at most C(199, 2) < 2^16, so they share a word
+        const uint32_t p0 = anchor_part(__builtin_amdgcn_readlane(key[ca], l));
+        const uint32_t p1 = (l + 1 < limit) ? anchor_part(__builtin_amdgcn_readlane(key[ca], l + 1)) : 0u;
+        const uint32_t total = wave_sum_dpp(p0 | (p1 << 16));
+        bad[ca] = (lane == l) ? (int)(total & 0xFFFFu) : ((lane == l + 1) ? (int)(total >> 16) : bad[ca]);
       }
     }
-    // acc = 1 (own zero-vector bin) + sum over the m-1 other anchors of cnt
     int sum = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
       if (k < m) {
-        const int bad = (acc[c] - m) >> 1;
-        sum += bad;
-        if (badness) badness[s * m + k] = bad;
+        sum += bad[c];
+        if (badness) badness[s * m + k] = bad[c];
       }
     }
     sum = wave_sum(sum);

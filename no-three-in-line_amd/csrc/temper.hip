@@ -22,6 +22,7 @@
 // (tempering.py:169-191), one wavefront per candidate pair; an accepted pair
 // swaps the slots' columns, badness, scores and removed counters.
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -51,20 +52,12 @@ struct WalkArgs {
   unsigned long long* proposals;
 };
 
+// One replica slot's `a.steps` proposals; `table` and `hist` are this
+// wavefront's LDS direction table and histogram.
 template <int CH>
-__global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* __restrict__ table_g,
-                                                   DirTableInfo info) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int waves = blockDim.x / kWave;
-  const int wave = threadIdx.x / kWave, lane = lane_id();
-  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
-  stage_table(table_g, table, info.entries);
-  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
-  __syncthreads();
-
-  const int slot = blockIdx.x * waves + wave;
-  if (slot >= a.replicas) return;
+__device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t* hist, const DirTableInfo& info,
+                             int slot) {
+  const int lane = lane_id();
   const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
   const uint32_t sweep = a.counters[0];
   const long long base = (long long)slot * m;
@@ -83,7 +76,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
     for (int c = 0; c < CH; ++c)
       if (lane + kWave * c >= m) key[c] = key0;
   }
-  const int trash = trash_word(info);
+  const int trash = trash_word(info, lane);
   int score = a.scores[slot];
   int removed = a.removed[slot];
   int best = a.best_scores[slot];
@@ -240,6 +233,58 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
   }
 }
 
+template <int CH>
+__global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* __restrict__ table_g,
+                                                   DirTableInfo info) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  stage_table(table_g, table, info.entries);
+  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+  const int slot = blockIdx.x * waves + wave;
+  if (slot < a.replicas) walk_replica<CH>(a, table, hist, info, slot);
+}
+
+// Several populations of different sizes in one launch: block b belongs to
+// the group whose block range holds b; each group has its own direction
+// table, histogram size and number of working waves per block.
+struct BatchGroup {
+  WalkArgs walk;
+  const uint16_t* table;
+  DirTableInfo info;
+  int ch;
+  int waves;
+  int block_begin;
+  int pad;
+};
+
+__global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __restrict__ groups, int group_count) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int g = 0;
+  while (g + 1 < group_count && (int)blockIdx.x >= groups[g + 1].block_begin) ++g;
+  const BatchGroup& G = groups[g];
+  const DirTableInfo info = G.info;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  stage_table(G.table, table, info.entries);
+  if (wave < G.waves)
+    for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+  if (wave >= G.waves) return;
+  const int slot = ((int)blockIdx.x - G.block_begin) * G.waves + wave;
+  if (slot >= G.walk.replicas) return;
+  switch (G.ch) {
+    case 1: walk_replica<1>(G.walk, table, hist, info, slot); break;
+    case 2: walk_replica<2>(G.walk, table, hist, info, slot); break;
+    case 3: walk_replica<3>(G.walk, table, hist, info, slot); break;
+    default: walk_replica<4>(G.walk, table, hist, info, slot); break;
+  }
+}
+
 struct ExchangeArgs {
   int n;
   int replicas;
@@ -257,8 +302,7 @@ struct ExchangeArgs {
 
 // One wavefront per candidate pair (a, a+1); pairs start at round % 2 inside
 // every ladder and do not overlap, so the swaps need no synchronisation.
-__global__ void __launch_bounds__(256) exchange_kernel(ExchangeArgs x) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+__device__ void exchange_pair(const ExchangeArgs& x, int wave) {
   const int lane = lane_id();
   const uint32_t round = x.counters[1];
   const int per_ladder = x.ladder / 2;  // candidate pairs per ladder (upper bound)
@@ -303,6 +347,36 @@ __global__ void __launch_bounds__(256) exchange_kernel(ExchangeArgs x) {
   }
 }
 
+__global__ void __launch_bounds__(256) exchange_kernel(ExchangeArgs x) {
+  exchange_pair(x, (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+}
+
+// Device-side description of a batched run (one allocation per plan).
+struct BatchTables {
+  BatchGroup walk[N3_MAX_GROUPS];
+  ExchangeArgs exchange[N3_MAX_GROUPS];
+  int exchange_begin[N3_MAX_GROUPS + 1];  // first exchange wave of each group
+  uint32_t* counters[N3_MAX_GROUPS];
+  uint32_t round_step[N3_MAX_GROUPS];     // 1 if the group exchanges
+  int count;
+};
+
+__global__ void __launch_bounds__(256) exchange_batched_kernel(const BatchTables* __restrict__ t) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  int g = 0;
+  while (g + 1 < t->count && wave >= t->exchange_begin[g + 1]) ++g;
+  if (wave >= t->exchange_begin[t->count] || t->round_step[g] == 0) return;
+  exchange_pair(t->exchange[g], wave - t->exchange_begin[g]);
+}
+
+__global__ void bump_batched_kernel(const BatchTables* __restrict__ t) {
+  const int g = threadIdx.x;
+  if (blockIdx.x == 0 && g < t->count) {
+    t->counters[g][0] += 1u;
+    t->counters[g][1] += t->round_step[g];
+  }
+}
+
 __global__ void bump_kernel(uint32_t* counters, uint32_t d_sweep, uint32_t d_round) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     counters[0] += d_sweep;
@@ -330,7 +404,7 @@ struct WalkPlan {
   void (*kernel)(WalkArgs, const uint16_t*, DirTableInfo);
 };
 
-static int walk_plan(int n, int replicas, WalkPlan* plan) {
+static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan) {
   int st = device_dirtable(n, &plan->table, &plan->info);
   if (st) return st;
   hipDeviceProp_t prop;
@@ -341,12 +415,18 @@ static int walk_plan(int n, int replicas, WalkPlan* plan) {
   int lds_limit = (int)prop.sharedMemPerBlock;
   if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
   const int table_bytes = table_lds_bytes(plan->info), hist_bytes = plan->info.hist_words * 4;
-  int waves = (lds_limit - table_bytes) / hist_bytes;
-  if (waves > 16) waves = 16;
-  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
-  // spread small populations over more CUs
-  const int per_cu = (replicas + prop.multiProcessorCount - 1) / prop.multiProcessorCount;
-  if (waves > per_cu) waves = per_cu < 1 ? 1 : per_cu;
+  const int fit = (lds_limit - table_bytes) / hist_bytes;
+  if (fit < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
+  int waves = fit < 16 ? fit : 16;
+  if (block_waves > 0) {
+    if (block_waves > waves)
+      return set_error(N3_EINVAL, "block_waves=%d exceeds the %d that fit in LDS for n=%d", block_waves, waves, n);
+    waves = block_waves;
+  } else {
+    // spread small populations over more CUs
+    const int per_cu = (replicas + prop.multiProcessorCount - 1) / prop.multiProcessorCount;
+    if (waves > per_cu) waves = per_cu < 1 ? 1 : per_cu;
+  }
   plan->waves = waves;
   plan->blocks = (unsigned)((replicas + waves - 1) / waves);
   plan->lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
@@ -453,9 +533,188 @@ static int capture_graph(const n3_chains* c, const WalkPlan& plan, int sweeps, i
   return st;
 }
 
+// ---------------------------------------------------------------------------
+// batched runs of several sizes
+// ---------------------------------------------------------------------------
+
+struct BatchPlan {
+  BatchTables host;
+  BatchTables* device = nullptr;
+  unsigned walk_blocks = 0;
+  int walk_threads = 0;
+  size_t walk_lds = 0;
+  unsigned exchange_blocks = 0;
+};
+
+static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
+  hipDeviceProp_t prop;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  int st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (st) return st;
+  int lds_limit = (int)prop.sharedMemPerBlock;
+  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
+  std::memset(&plan->host, 0, sizeof(BatchTables));
+  // largest sizes first: their blocks run longest, so they are dispatched first
+  int order[N3_MAX_GROUPS];
+  for (int i = 0; i < count; ++i) order[i] = i;
+  for (int i = 1; i < count; ++i)
+    for (int j = i; j > 0 && groups[order[j]].chains.n > groups[order[j - 1]].chains.n; --j) {
+      const int t = order[j];
+      order[j] = order[j - 1];
+      order[j - 1] = t;
+    }
+  int blocks = 0, max_waves = 1, exchange_waves = 0;
+  size_t lds = 0;
+  for (int i = 0; i < count; ++i) {
+    const n3_group& gr = groups[order[i]];
+    const n3_chains* c = &gr.chains;
+    BatchGroup& bg = plan->host.walk[i];
+    st = device_dirtable(c->n, &bg.table, &bg.info);
+    if (st) return st;
+    const int table_bytes = table_lds_bytes(bg.info), hist_bytes = bg.info.hist_words * 4;
+    const int fit = (lds_limit - table_bytes) / hist_bytes;
+    int waves = fit < 16 ? fit : 16;
+    if (c->block_waves > 0) {
+      if (c->block_waves > waves)
+        return set_error(N3_EINVAL, "block_waves=%d exceeds the %d that fit in LDS for n=%d", c->block_waves, waves,
+                         c->n);
+      waves = c->block_waves;
+    }
+    if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", c->n);
+    bg.walk = walk_args(c, gr.steps, gr.bad_bias, gr.seed);
+    bg.ch = (2 * c->n + kWave - 1) / kWave;
+    bg.waves = waves;
+    bg.block_begin = blocks;
+    blocks += (c->replicas + waves - 1) / waves;
+    if (waves > max_waves) max_waves = waves;
+    const size_t need = (size_t)table_bytes + (size_t)waves * hist_bytes;
+    if (need > lds) lds = need;
+    plan->host.exchange[i] = ExchangeArgs{c->n, c->replicas, gr.ladder > 1 ? gr.ladder : 2, (uint32_t)gr.seed,
+                                          (uint32_t)(gr.seed >> 32), c->columns, c->badness, c->scores,
+                                          c->removed_points, c->exchange_thresholds, c->exchange_sign,
+                                          c->k_exchange, c->counters};
+    plan->host.exchange_begin[i] = exchange_waves;
+    if (gr.ladder > 1) exchange_waves += (c->replicas / gr.ladder) * (gr.ladder / 2);
+    plan->host.counters[i] = c->counters;
+    plan->host.round_step[i] = gr.ladder > 1 ? 1u : 0u;
+  }
+  plan->host.exchange_begin[count] = exchange_waves;
+  plan->host.count = count;
+  plan->walk_blocks = (unsigned)blocks;
+  plan->walk_threads = max_waves * kWave;
+  plan->walk_lds = lds;
+  plan->exchange_blocks = (unsigned)((exchange_waves + 3) / 4);
+  st = hip_status(hipMalloc(&plan->device, sizeof(BatchTables)), "hipMalloc(batch tables)");
+  if (st) return st;
+  st = hip_status(hipMemcpy(plan->device, &plan->host, sizeof(BatchTables), hipMemcpyHostToDevice),
+                  "hipMemcpy(batch tables)");
+  if (st) {
+    (void)hipFree(plan->device);
+    plan->device = nullptr;
+  }
+  return st;
+}
+
+static int enqueue_batched_sweep(const BatchPlan& plan, hipStream_t stream) {
+  // the group array is the first member of the device tables
+  static_assert(offsetof(BatchTables, walk) == 0, "walk groups lead the batch tables");
+  hipLaunchKernelGGL(walk_batched_kernel, dim3(plan.walk_blocks), dim3(plan.walk_threads), plan.walk_lds, stream,
+                     reinterpret_cast<const BatchGroup*>(plan.device), plan.host.count);
+  int st = hip_status(hipGetLastError(), "walk_batched_kernel launch");
+  if (st) return st;
+  if (plan.exchange_blocks > 0) {
+    hipLaunchKernelGGL(exchange_batched_kernel, dim3(plan.exchange_blocks), dim3(256), 0, stream, plan.device);
+    st = hip_status(hipGetLastError(), "exchange_batched_kernel launch");
+    if (st) return st;
+  }
+  hipLaunchKernelGGL(bump_batched_kernel, dim3(1), dim3(64), 0, stream, plan.device);
+  return hip_status(hipGetLastError(), "bump_batched_kernel launch");
+}
+
+struct BatchEntry {
+  std::vector<unsigned char> key;
+  BatchPlan plan;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int sweeps_per_graph;
+};
+
+static std::vector<BatchEntry> g_batches;
+
 }  // namespace no3
 
 using namespace no3;
+
+extern "C" int n3_temper_batched_run(const n3_group* groups, int group_count, int sweeps, void* stream) {
+  if (!groups || group_count < 1 || group_count > N3_MAX_GROUPS)
+    return set_error(N3_EINVAL, "group_count must be in [1, %d]", N3_MAX_GROUPS);
+  if (sweeps < 0) return set_error(N3_EINVAL, "negative sweeps");
+  for (int i = 0; i < group_count; ++i) {
+    int st = check_chains(&groups[i].chains);
+    if (st) return st;
+    if (groups[i].steps < 1 || groups[i].bad_bias < 0 || groups[i].bad_bias > 100)
+      return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
+    if (groups[i].ladder > 1) {
+      st = check_exchange(&groups[i].chains, groups[i].ladder);
+      if (st) return st;
+    }
+  }
+  if (sweeps == 0) return N3_OK;
+  hipStream_t s = as_stream(stream);
+  const int per_graph = sweeps < 16 ? sweeps : 16;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  std::vector<unsigned char> key(sizeof(n3_group) * group_count + 3 * sizeof(intptr_t));
+  std::memcpy(key.data(), groups, sizeof(n3_group) * group_count);
+  const intptr_t extra[3] = {(intptr_t)s, (intptr_t)device, (intptr_t)per_graph};
+  std::memcpy(key.data() + sizeof(n3_group) * group_count, extra, sizeof(extra));
+  std::lock_guard<std::mutex> lock(g_graph_mutex);
+  BatchEntry* entry = nullptr;
+  for (auto& e : g_batches)
+    if (e.key == key) entry = &e;
+  if (!entry) {
+    BatchEntry e;
+    e.key = key;
+    e.sweeps_per_graph = per_graph;
+    int st = batch_plan(groups, group_count, &e.plan);
+    if (st) return st;
+    hipStream_t cap = nullptr;
+    st = hip_status(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+    if (st) return st;
+    st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (!st) {
+      for (int i = 0; i < per_graph && !st; ++i) st = enqueue_batched_sweep(e.plan, cap);
+      hipGraph_t g = nullptr;
+      const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
+      if (!st) st = end;
+      if (!st) {
+        st = hip_status(hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+        e.graph = g;
+        if (st) (void)hipGraphDestroy(g);
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+    }
+    (void)hipStreamDestroy(cap);
+    if (st) {
+      (void)hipFree(e.plan.device);
+      return st;
+    }
+    g_batches.push_back(std::move(e));
+    entry = &g_batches.back();
+  }
+  const int full = sweeps / entry->sweeps_per_graph, rest = sweeps % entry->sweeps_per_graph;
+  for (int i = 0; i < full; ++i) {
+    const int st = hip_status(hipGraphLaunch(entry->exec, s), "hipGraphLaunch");
+    if (st) return st;
+  }
+  for (int i = 0; i < rest; ++i) {
+    const int st = enqueue_batched_sweep(entry->plan, s);
+    if (st) return st;
+  }
+  return N3_OK;
+}
 
 extern "C" int n3_temper_advance(const n3_chains* chains, int steps, int bad_bias, uint64_t seed, void* stream) {
   int st = check_chains(chains);
@@ -463,7 +722,7 @@ extern "C" int n3_temper_advance(const n3_chains* chains, int steps, int bad_bia
   if (steps < 1 || bad_bias < 0 || bad_bias > 100)
     return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
   WalkPlan plan;
-  st = walk_plan(chains->n, chains->replicas, &plan);
+  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan);
   if (st) return st;
   return enqueue_advance(chains, plan, steps, bad_bias, seed, as_stream(stream));
 }
@@ -489,7 +748,7 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
   }
   if (sweeps == 0) return N3_OK;
   WalkPlan plan;
-  st = walk_plan(chains->n, chains->replicas, &plan);
+  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan);
   if (st) return st;
   hipStream_t s = as_stream(stream);
   const int per_graph = sweeps < 32 ? sweeps : 32;
@@ -526,6 +785,12 @@ extern "C" void n3_temper_release(void) {
     (void)hipGraphDestroy(e.graph);
   }
   g_graphs.clear();
+  for (auto& e : g_batches) {
+    (void)hipGraphExecDestroy(e.exec);
+    (void)hipGraphDestroy(e.graph);
+    (void)hipFree(e.plan.device);
+  }
+  g_batches.clear();
 }
 
 // Host-side threshold tables shared by the device kernels and any CPU replay.

@@ -49,11 +49,11 @@ __device__ inline void direction_counts(const uint16_t* table, uint32_t* hist, i
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     word[c] = include[c] ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
-    atomicAdd(&hist[word[c]], 1u << (((uint32_t)id[c] << 3) & 31u));
+    atomicAdd(&hist[word[c]], 1u << ((uint32_t)id[c] << 3));
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < CH; ++c) cnt[c] = (int)((hist[word[c]] >> (((uint32_t)id[c] << 3) & 31u)) & 0xFFu);
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], (uint32_t)id[c] << 3, 8);
   wave_sync();
 #pragma unroll
   for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
@@ -68,11 +68,11 @@ __device__ inline void direction_counts_known(uint32_t* hist, int trash, const i
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     word[c] = include[c] ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
-    atomicAdd(&hist[word[c]], 1u << (((uint32_t)id[c] << 3) & 31u));
+    atomicAdd(&hist[word[c]], 1u << ((uint32_t)id[c] << 3));
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < CH; ++c) cnt[c] = (int)((hist[word[c]] >> (((uint32_t)id[c] << 3) & 31u)) & 0xFFu);
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], (uint32_t)id[c] << 3, 8);
   wave_sync();
 #pragma unroll
   for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;

@@ -101,6 +101,9 @@ class TemperingChains:
         # sweep, round, exchanges (64-bit); valid proposals, accepted moves
         self.counters = torch.zeros(4, dtype=torch.int32, device=dev)
         self.proposals = torch.zeros(2, dtype=torch.int64, device=dev)
+        # replicas (wavefronts) per workgroup of the walk; 0 lets the library
+        # choose (see autotune.py for the measured choice per n)
+        self.block_waves = 0
 
     # -- helpers -----------------------------------------------------------
     def _rescore(self, columns: torch.Tensor, scores: torch.Tensor, badness: torch.Tensor,
@@ -141,6 +144,7 @@ class TemperingChains:
             c.k_exchange = k
         c.counters = self.counters.data_ptr()
         c.proposals = self.proposals.data_ptr()
+        c.block_waves = int(self.block_waves)
         return c
 
     def _check_umbrella(self) -> None:

@@ -117,6 +117,10 @@ typedef struct n3_chains {
     uint32_t* counters;      /* [4]: sweep, round, exchanges (low), exchanges (high) */
     int64_t* proposals;      /* [2]: valid proposals (total), accepted moves (total) */
     int block_waves;         /* replicas (wavefronts) per workgroup; 0 = automatic */
+    int32_t* log;            /* optional proposal log (NULL = off): per valid proposal
+                                (badness u, badness v, energy, slot, delta, accepted) */
+    int64_t log_capacity;    /* entries */
+    uint64_t* log_count;     /* entries written (may exceed the capacity) */
 } n3_chains;
 
 /* `steps` proposals per replica (one sweep), TemperingChains.advance
@@ -162,6 +166,55 @@ int n3_temper_batched_run(const n3_group* groups, int group_count, int sweeps, v
 
 /* Drops cached graphs (all of them). */
 void n3_temper_release(void);
+
+/* ---- learned move proposals (no reference counterpart) ----------------- */
+
+/* The proposal network: 8 features -> 128 hidden (ReLU) -> 1 logit.  Its
+ * parameters are one bf16 vector of n3_mlp_params() entries laid out as
+ * W1[128][8] | b1[128] | w2[128] | b2; every product accumulates in fp32 on
+ * MFMA (v_mfma_f32_32x32x8_bf16). */
+#define N3_MLP_INPUTS 8
+#define N3_MLP_HIDDEN 128
+int n3_mlp_params(void);
+
+/* Features (bf16[8]) of a candidate point with collinear-pair count `count`
+ * at temperature exp(log_temperature) in a state of energy `energy`.  Host. */
+int n3_mlp_features(int count, float log_temperature, int energy, uint16_t* host_out);
+
+/* Logits of `rows` feature rows x (bf16[rows][8]) into out (fp32[rows]). */
+int n3_mlp_forward(const uint16_t* params, const uint16_t* x, int64_t rows, float* out, void* stream);
+
+/* Gradient of the mean pair loss BCE(sigmoid(g(xu) + g(xv) - b2), y) over
+ * `pairs` tuples, ADDED to grad (fp32[n3_mlp_params()]); the summed loss is
+ * added to loss[0]. */
+int n3_mlp_grad(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y, int64_t pairs,
+                float* grad, float* loss, void* stream);
+
+/* One Adam step on the fp32 master copy; writes the bf16 parameters. */
+int n3_mlp_adam(float* master, uint16_t* params, const float* grad, float* m, float* v, int step, float lr,
+                float beta1, float beta2, float eps, void* stream);
+
+/* Integer proposal weights table[t][e][b] = clamp(round(4096 exp(clamp(g,
+ * -6, 6))), 1, 2^21) of the network's logit g for temperature index t
+ * (log_temps[t]), energy e in [0, max_energy] and count b in [0, 63]. */
+int n3_mlp_weight_table(const uint16_t* params, const float* log_temps, int temps, int max_energy, uint32_t* table,
+                        void* stream);
+
+/* host_out[r][d + k] = floor(2^32 exp(-d / T_r)), saturating at 2^63, for
+ * d in [-k, k): the acceptance words of the learned walk. */
+int n3_metropolis_thresholds64(const float* host_temperatures, int replicas, int k, uint64_t* host_out);
+
+/* `sweeps` x (learned-proposal walk of `steps` proposals; exchange(ladder)
+ * if ladder > 1), graph-captured.  Slot r's first moved point is drawn from
+ * 0.2 uniform + 0.8 weights[temp_index[r]][E][badness], the second uniformly
+ * from the others; acceptance is the Metropolis-Hastings test on the exact
+ * integer delta, in 128-bit integer arithmetic (see csrc/learned.hip). */
+int n3_temper_run_learned(const n3_chains* chains, const uint32_t* weights, const int32_t* temp_index,
+                          int max_energy, const uint64_t* thr64, int k64, int sweeps, int steps, int ladder,
+                          uint64_t seed, void* stream);
+
+/* Drops the learned walk's cached graphs. */
+void n3_learned_release(void);
 
 /* Host tables of the integer acceptance tests (no device work).
  * Metropolis: host_out[r][d] = floor(2^32 exp(-d / T_r)) for 1 <= d < k

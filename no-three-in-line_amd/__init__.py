@@ -25,12 +25,12 @@ from .gpu_delta import (
 from .state import PairPermutations, points_to_permutations, random_state
 from .tempering import TemperingChains
 from .batched import BatchGroup, run_batched, seeded_chains
-from . import autotune
+from . import autotune, learned
 
 __all__ = [
     "SCORE_AUTO", "SCORE_DETERMINANT", "SCORE_SLOPE", "SENTINEL32", "SENTINEL64",
     "check_status", "generate_states", "score_population_badness", "score_population_device", "score_population_gpu",
     "delta_table", "delta_table_device", "evaluate_transpositions", "evaluate_two_switches", "score_two_switches_gpu",
     "PairPermutations", "points_to_permutations", "random_state", "TemperingChains", "_lib",
-    "BatchGroup", "run_batched", "seeded_chains", "autotune",
+    "BatchGroup", "run_batched", "seeded_chains", "autotune", "learned",
 ]

@@ -52,6 +52,9 @@ class Chains(ctypes.Structure):
         ("counters", ctypes.c_void_p),
         ("proposals", ctypes.c_void_p),
         ("block_waves", ctypes.c_int),
+        ("log", ctypes.c_void_p),
+        ("log_capacity", ctypes.c_int64),
+        ("log_count", ctypes.c_void_p),
     ]
 
 
@@ -94,6 +97,16 @@ SIGNATURES = {
     "n3_temper_batched_run": (_i, [ctypes.POINTER(Group), _i, _i, _vp]),
     "n3_temper_release": (None, []),
     "n3_metropolis_thresholds": (_i, [_vp, _i, _i, _vp]),
+    "n3_mlp_params": (_i, []),
+    "n3_mlp_features": (_i, [_i, ctypes.c_float, _i, _vp]),
+    "n3_mlp_forward": (_i, [_vp, _vp, _i64, _vp, _vp]),
+    "n3_mlp_grad": (_i, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "n3_mlp_adam": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_float, _vp]),
+    "n3_mlp_weight_table": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "n3_metropolis_thresholds64": (_i, [_vp, _i, _i, _vp]),
+    "n3_temper_run_learned": (_i, [ctypes.POINTER(Chains), _vp, _vp, _i, _vp, _i, _i, _i, _i, _u64, _vp]),
+    "n3_learned_release": (None, []),
     "n3_exchange_thresholds": (_i, [_vp, _i, _i, _i, _vp, _vp]),
 }
 

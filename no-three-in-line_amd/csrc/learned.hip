@@ -1,0 +1,435 @@
+// Tempering walk with learned move proposals.
+//
+// Proposal: the first slot u comes from the mixture
+//   pi_x(k) = 0.2 / m + 0.8 w_x(k) / S1_x,   S1_x = sum_k w_x(k),
+// where w_x(k) = W[t][E(x)][badness_x(k)] is the proposal network's tabulated
+// integer weight for slot k (temperature index t of the replica slot, energy
+// E(x), the slot's collinear-pair count); the second slot v is uniform over
+// the other m - 1.  The unordered move {u, v} has probability
+//   q(x -> x') = (pi_x(u) + pi_x(v)) / (m - 1),
+// and the reverse move from x' is the same pair, so the Hastings ratio is
+//   q(x' -> x) / q(x -> x') = (N' S1) / (N S1'),  N = 2 S1 + 4 m (w(u) + w(v))
+// (pi = (S1 + 4 m w) / (5 m S1)).  Acceptance stays on the exact integer delta:
+//   accept  iff  r * N * S1' < thr64(delta) * N' * S1
+// with r the move's 32-bit Philox word and thr64(d) = floor(2^32 exp(-d/T))
+// (saturating at 2^63), evaluated in 128-bit integers -- so a CPU replay with
+// the same weight table reproduces every trajectory bit for bit.  Evaluating
+// the reverse probability needs the badness of every slot in x', so every
+// valid proposal runs the four direction histograms that the uniform walk
+// only needs after an acceptance.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "temper_internal.h"
+#include "wave_ops.h"
+
+namespace no3 {
+
+constexpr uint32_t kTagLearn = 0x4C524E44u;  // "LRND"
+constexpr uint32_t kUniformShare = 858993459u;  // floor(0.2 * 2^32)
+
+struct LearnedArgs {
+  const uint32_t* weights;             // [temps][max_energy + 1][64]
+  const int32_t* temp_index;           // [R]
+  int max_energy;
+  const unsigned long long* thr64;     // [R][2 k64], entry d + k64
+  int k64;
+};
+
+__device__ inline uint32_t weight_of(uint32_t row, int count) { return (uint32_t)__shfl((int)row, count < 63 ? count : 63, kWave); }
+
+template <int CH>
+__device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, const uint16_t* table, uint32_t* hist,
+                                     const DirTableInfo& info, int slot) {
+  const int lane = lane_id();
+  const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
+  const uint32_t sweep = a.counters[0];
+  const long long base = (long long)slot * m;
+
+  int key[CH], bad[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    key[c] = k < m ? (k >> 1) * width + a.columns[base + k] : 0;
+    bad[c] = k < m ? a.badness[base + k] : 0;
+  }
+  {
+    const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (lane + kWave * c >= m) key[c] = key0;
+  }
+  const int trash = trash_word(info, lane);
+  int score = a.scores[slot];
+  int removed = a.removed[slot];
+  int best = a.best_scores[slot];
+  int accepted = 0;
+  unsigned long long valid_moves = 0;
+  const uint32_t* wt = L.weights + (long long)L.temp_index[slot] * (L.max_energy + 1) * 64;
+  const unsigned long long* thr = L.thr64 + (long long)slot * 2 * L.k64;
+  auto weight_row = [&](int energy) -> uint32_t {
+    const int e = energy < 0 ? 0 : (energy > L.max_energy ? L.max_energy : energy);
+    return wt[(long long)e * 64 + lane];
+  };
+  int w[CH];
+  uint32_t s1;
+  {
+    const uint32_t row = weight_row(score);
+    uint32_t part = 0;
+    // the row exchange runs on every lane (a permute from a lane outside the
+    // exec mask reads nothing); idle lanes are zeroed afterwards
+#pragma unroll
+    for (int c = 0; c < CH; ++c) w[c] = (int)weight_of(row, bad[c]);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (lane + kWave * c >= m) w[c] = 0;
+      part += (uint32_t)w[c];
+    }
+    s1 = wave_sum_dpp(part);
+  }
+
+  for (int step = 0; step < a.steps; ++step) {
+    uint32_t r[4];
+    philox4x32_10((uint32_t)step, sweep, (uint32_t)slot, kTagLearn, a.seed_lo, a.seed_hi, r);
+    int u;
+    if (r[0] < kUniformShare) {
+      u = (int)mulhi32(r[1], (uint32_t)m);
+    } else {
+      // inverse CDF of w over the slots in slot order
+      const uint32_t target = (uint32_t)(((uint64_t)r[1] * (uint64_t)s1) >> 32);
+      int below = 0;
+      uint32_t carry = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        uint32_t x = (uint32_t)w[c];
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const uint32_t y = (uint32_t)__shfl_up((int)x, off, kWave);
+          if (lane >= off) x += y;
+        }
+        const int k = lane + kWave * c;
+        below += __popcll(__ballot(k < m && carry + x <= target));
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+      }
+      u = below < m - 1 ? below : m - 1;
+    }
+    const int vv = (int)mulhi32(r[2], (uint32_t)(m - 1));
+    const int v = vv + (vv >= u ? 1 : 0);
+    const int ru = u >> 1, rv = v >> 1;
+    const int key_u = slot_value<CH>(key, u), key_v = slot_value<CH>(key, v);
+    const int cu = key_u - ru * width, cv = key_v - rv * width;
+    const int cu_other = slot_value<CH>(key, u ^ 1) - ru * width;
+    const int cv_other = slot_value<CH>(key, v ^ 1) - rv * width;
+    const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
+    if (!valid) continue;
+    valid_moves += 1;
+    const int ur = 2 * ru, vr = 2 * rv;
+    const int dd = (cu == a.reference[ur] || cu == a.reference[ur + 1]) +
+                   (cv == a.reference[vr] || cv == a.reference[vr + 1]) -
+                   (cv == a.reference[ur] || cv == a.reference[ur + 1]) -
+                   (cu == a.reference[vr] || cu == a.reference[vr + 1]);
+
+    const int id_uv = table[key_v - key_u + offset];
+    int idu[CH];
+    bool others[CH];
+    uint64_t on_uv[CH];
+    int joint_old = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      others[c] = (k < m) && k != u && k != v;
+      idu[c] = table[key[c] - key_u + offset];
+      on_uv[c] = __ballot(others[c] && idu[c] == id_uv);
+      joint_old += __popcll(on_uv[c]);
+    }
+    const int key_u2 = ru * width + cv, key_v2 = rv * width + cu;
+    const int id_u2v2 = table[key_v2 - key_u2 + offset];
+    int keys[CH], cnt_u2[CH], cnt_v2[CH], cnt_u[CH], cnt_v[CH], id[CH];
+    bool include[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      include[c] = (k < m) && k != u;
+      keys[c] = (k == v) ? key_v2 : key[c];
+    }
+    direction_counts<CH>(table, hist, trash, offset, key_u2, keys, include, cnt_u2, id);
+    uint32_t part = 0;
+    int joint_new = 0;
+    uint64_t on_u2v2[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (include[c]) part += (uint32_t)(cnt_u2[c] - 1);
+      on_u2v2[c] = __ballot(others[c] && id[c] == id_u2v2);
+      joint_new += __popcll(on_u2v2[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      include[c] = (k < m) && k != v;
+      keys[c] = (k == u) ? key_u2 : key[c];
+    }
+    direction_counts<CH>(table, hist, trash, offset, key_v2, keys, include, cnt_v2, id);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (include[c]) part += (uint32_t)(cnt_v2[c] - 1) << 16;
+    const uint32_t packed = wave_sum_dpp(part);
+    const int bad_u2 = (int)((packed & 0xFFFFu) >> 1), bad_v2 = (int)((packed >> 16) >> 1);
+    const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
+    const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
+
+    // badness of every slot in the moved state (needed for the reverse move)
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
+    direction_counts_known<CH>(hist, trash, idu, include, cnt_u);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+    direction_counts<CH>(table, hist, trash, offset, key_v, key, include, cnt_v, id);
+    int nb[CH], w2[CH];
+    const uint32_t row2 = weight_row(score + delta);
+    uint32_t part2 = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      const int bit_old = (int)((on_uv[c] >> lane) & 1ull);
+      const int bit_new = (int)((on_u2v2[c] >> lane) & 1ull);
+      nb[c] = bad[c];
+      if (others[c]) nb[c] += (cnt_u2[c] + cnt_v2[c] - bit_new) - (cnt_u[c] + cnt_v[c] - bit_old);
+      if (k == u) nb[c] = bad_u2;
+      if (k == v) nb[c] = bad_v2;
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) w2[c] = (int)weight_of(row2, nb[c]);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (lane + kWave * c >= m) w2[c] = 0;
+      part2 += (uint32_t)w2[c];
+    }
+    const uint32_t s1_new = wave_sum_dpp(part2);
+    const uint64_t wu = (uint32_t)slot_value<CH>(w, u), wv = (uint32_t)slot_value<CH>(w, v);
+    const uint64_t wu2 = (uint32_t)slot_value<CH>(w2, u), wv2 = (uint32_t)slot_value<CH>(w2, v);
+    const uint64_t big_n = 2ull * s1 + 4ull * (uint64_t)m * (wu + wv);
+    const uint64_t big_n2 = 2ull * s1_new + 4ull * (uint64_t)m * (wu2 + wv2);
+    const int d = delta < -L.k64 ? -L.k64 : (delta > L.k64 - 1 ? L.k64 - 1 : delta);
+    const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * s1_new;
+    const unsigned __int128 rhs = (unsigned __int128)thr[d + L.k64] * big_n2 * s1;
+    if (!(lhs < rhs)) continue;
+
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      bad[c] = nb[c];
+      w[c] = w2[c];
+      if (k == u) key[c] = key_u2;
+      if (k == v) key[c] = key_v2;
+    }
+    s1 = s1_new;
+    score += delta;
+    removed += dd;
+    accepted += 1;
+    if (score < best) {
+      best = score;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        if (k < m) a.best_columns[base + k] = key[c] - (k >> 1) * width;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    if (k < m) {
+      a.columns[base + k] = key[c] - (k >> 1) * width;
+      a.badness[base + k] = bad[c];
+    }
+  }
+  if (lane == 0) {
+    a.scores[slot] = score;
+    a.removed[slot] = removed;
+    a.best_scores[slot] = best;
+    a.accepted[slot] += accepted;
+    atomicAdd(&a.proposals[0], valid_moves);
+    atomicAdd(&a.proposals[1], (unsigned long long)accepted);
+  }
+}
+
+template <int CH>
+__global__ void __launch_bounds__(1024) walk_learned_kernel(WalkArgs a, LearnedArgs L,
+                                                           const uint16_t* __restrict__ table_g, DirTableInfo info) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  stage_table(table_g, table, info.entries);
+  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+  const int slot = blockIdx.x * waves + wave;
+  if (slot < a.replicas) walk_replica_learned<CH>(a, L, table, hist, info, slot);
+}
+
+__global__ void bump_sweep_kernel(uint32_t* counters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counters[0] += 1u;
+}
+
+struct LearnedPlan {
+  const uint16_t* table;
+  DirTableInfo info;
+  int waves;
+  unsigned blocks;
+  size_t lds;
+  void (*kernel)(WalkArgs, LearnedArgs, const uint16_t*, DirTableInfo);
+};
+
+static int learned_plan(const n3_chains* c, LearnedPlan* plan) {
+  int st = device_dirtable(c->n, &plan->table, &plan->info);
+  if (st) return st;
+  hipDeviceProp_t prop;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (st) return st;
+  int lds_limit = (int)prop.sharedMemPerBlock;
+  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
+  const int table_bytes = table_lds_bytes(plan->info), hist_bytes = plan->info.hist_words * 4;
+  const int fit = (lds_limit - table_bytes) / hist_bytes;
+  if (fit < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", c->n);
+  int waves = fit < 16 ? fit : 16;
+  if (c->block_waves > 0) {
+    if (c->block_waves > waves)
+      return set_error(N3_EINVAL, "block_waves=%d exceeds the %d that fit in LDS for n=%d", c->block_waves, waves,
+                       c->n);
+    waves = c->block_waves;
+  } else {
+    const int per_cu = (c->replicas + prop.multiProcessorCount - 1) / prop.multiProcessorCount;
+    if (waves > per_cu) waves = per_cu < 1 ? 1 : per_cu;
+  }
+  plan->waves = waves;
+  plan->blocks = (unsigned)((c->replicas + waves - 1) / waves);
+  plan->lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
+  switch ((2 * c->n + kWave - 1) / kWave) {
+    case 1: plan->kernel = walk_learned_kernel<1>; break;
+    case 2: plan->kernel = walk_learned_kernel<2>; break;
+    case 3: plan->kernel = walk_learned_kernel<3>; break;
+    default: plan->kernel = walk_learned_kernel<4>; break;
+  }
+  return N3_OK;
+}
+
+static int enqueue_learned(const n3_chains* c, const LearnedPlan& plan, const LearnedArgs& L, int steps, uint64_t seed,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream,
+                     walk_args(c, steps, 0, seed), L, plan.table, plan.info);
+  int st = hip_status(hipGetLastError(), "walk_learned_kernel launch");
+  if (st) return st;
+  hipLaunchKernelGGL(bump_sweep_kernel, dim3(1), dim3(64), 0, stream, c->counters);
+  return hip_status(hipGetLastError(), "bump_sweep_kernel launch");
+}
+
+struct LearnedEntry {
+  std::vector<unsigned char> key;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int per_graph;
+};
+static std::mutex g_learned_mutex;
+static std::vector<LearnedEntry> g_learned;
+
+}  // namespace no3
+
+using namespace no3;
+
+extern "C" int n3_metropolis_thresholds64(const float* host_temperatures, int replicas, int k, uint64_t* host_out) {
+  if (!host_temperatures || !host_out || replicas < 1 || k < 2) return set_error(N3_EINVAL, "bad threshold arguments");
+  const double cap = 9223372036854775808.0;  // 2^63
+  for (int r = 0; r < replicas; ++r) {
+    const double t = (double)host_temperatures[r];
+    if (!(t > 0.0) || !std::isfinite(t)) return set_error(N3_EINVAL, "temperatures must be positive and finite");
+    for (int d = -k; d < k; ++d) {
+      const double x = std::ldexp(std::exp(-(double)d / t), 32);
+      host_out[(long long)r * 2 * k + (d + k)] = x >= cap ? (uint64_t)1 << 63 : (uint64_t)std::floor(x);
+    }
+  }
+  return N3_OK;
+}
+
+extern "C" int n3_temper_run_learned(const n3_chains* chains, const uint32_t* weights, const int32_t* temp_index,
+                                     int max_energy, const uint64_t* thr64, int k64, int sweeps, int steps, int ladder,
+                                     uint64_t seed, void* stream) {
+  int st = check_chains(chains);
+  if (st) return st;
+  if (!weights || !temp_index || !thr64 || max_energy < 0 || k64 < 2 || steps < 1 || sweeps < 0)
+    return set_error(N3_EINVAL, "bad learned-walk arguments");
+  if (ladder > 1) {
+    st = check_exchange(chains, ladder);
+    if (st) return st;
+  }
+  if (sweeps == 0) return N3_OK;
+  LearnedPlan plan;
+  st = learned_plan(chains, &plan);
+  if (st) return st;
+  const LearnedArgs L{weights, temp_index, max_energy, reinterpret_cast<const unsigned long long*>(thr64), k64};
+  hipStream_t s = as_stream(stream);
+  const int per_graph = sweeps < 16 ? sweeps : 16;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  std::vector<unsigned char> key(sizeof(n3_chains) + sizeof(LearnedArgs) + 6 * sizeof(intptr_t));
+  std::memcpy(key.data(), chains, sizeof(n3_chains));
+  std::memcpy(key.data() + sizeof(n3_chains), &L, sizeof(LearnedArgs));
+  const intptr_t extra[6] = {(intptr_t)s, (intptr_t)device, per_graph, steps, ladder, (intptr_t)seed};
+  std::memcpy(key.data() + sizeof(n3_chains) + sizeof(LearnedArgs), extra, sizeof(extra));
+  std::lock_guard<std::mutex> lock(g_learned_mutex);
+  LearnedEntry* entry = nullptr;
+  for (auto& e : g_learned)
+    if (e.key == key) entry = &e;
+  if (!entry) {
+    LearnedEntry e;
+    e.key = key;
+    e.per_graph = per_graph;
+    hipStream_t cap = nullptr;
+    st = hip_status(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+    if (st) return st;
+    st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (!st) {
+      for (int i = 0; i < per_graph && !st; ++i) {
+        st = enqueue_learned(chains, plan, L, steps, seed, cap);
+        if (!st && ladder > 1) st = enqueue_exchange(chains, ladder, seed, cap);
+      }
+      hipGraph_t g = nullptr;
+      const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
+      if (!st) st = end;
+      if (!st) {
+        st = hip_status(hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+        e.graph = g;
+        if (st) (void)hipGraphDestroy(g);
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+    }
+    (void)hipStreamDestroy(cap);
+    if (st) return st;
+    g_learned.push_back(e);
+    entry = &g_learned.back();
+  }
+  const int full = sweeps / entry->per_graph, rest = sweeps % entry->per_graph;
+  for (int i = 0; i < full; ++i) {
+    st = hip_status(hipGraphLaunch(entry->exec, s), "hipGraphLaunch");
+    if (st) return st;
+  }
+  for (int i = 0; i < rest && !st; ++i) {
+    st = enqueue_learned(chains, plan, L, steps, seed, s);
+    if (!st && ladder > 1) st = enqueue_exchange(chains, ladder, seed, s);
+  }
+  return st;
+}
+
+extern "C" void n3_learned_release(void) {
+  std::lock_guard<std::mutex> lock(g_learned_mutex);
+  for (auto& e : g_learned) {
+    (void)hipGraphExecDestroy(e.exec);
+    (void)hipGraphDestroy(e.graph);
+  }
+  g_learned.clear();
+}

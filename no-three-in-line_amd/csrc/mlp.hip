@@ -1,0 +1,371 @@
+// Move-proposal network: 8 features -> 128 hidden (ReLU) -> 1 logit, BF16
+// parameters, FP32 accumulation, hand-written MFMA kernels for inference,
+// training gradients and the tabulated proposal weights.
+//
+// Parameters are one bf16 vector of kParams entries:
+//   W1[128][8] | b1[128] | w2[128] | b2
+// Inference on a tile of 32 rows is H^T = W1 X^T with four
+// v_mfma_f32_32x32x8_bf16 (M = hidden, N = rows, K = features), so each lane
+// ends up holding one row's pre-activations for 16 hidden units per tile and
+// the output layer is a per-lane dot product plus one cross-half exchange.
+// The weight gradient dW1 = dH X is a second MFMA product with the rows as K.
+#include "runtime.h"
+#include "wave_ops.h"
+
+namespace no3 {
+
+constexpr int kIn = 8;
+constexpr int kHidden = 128;
+constexpr int kW1 = 0;
+constexpr int kB1 = kHidden * kIn;
+constexpr int kW2 = kB1 + kHidden;
+constexpr int kB2 = kW2 + kHidden;
+constexpr int kParams = kB2 + 1;  // 1281
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline float bf16_to_float(uint16_t h) {
+  union {
+    uint32_t u;
+    float f;
+  } v;
+  v.u = (uint32_t)h << 16;
+  return v.f;
+}
+
+// round to nearest even (finite inputs)
+__host__ __device__ inline uint16_t float_to_bf16(float f) {
+  union {
+    uint32_t u;
+    float f;
+  } v;
+  v.f = f;
+  const uint32_t lsb = (v.u >> 16) & 1u;
+  return (uint16_t)((v.u + 0x7FFFu + lsb) >> 16);
+}
+
+// Features of a candidate point with collinear-pair count b at temperature
+// exp(log_t) in a state of energy e: b in four encodings, the temperature,
+// the energy in two encodings and one interaction term.
+__host__ __device__ inline void point_features(int b, float log_t, int e, float f[kIn]) {
+  const float fb = (float)b;
+  const float fe = (float)e;
+  f[0] = fb * 0.125f;
+  f[1] = sqrtf(fb) * (1.0f / 3.0f);
+  f[2] = logf(1.0f + fb) * (1.0f / 3.0f);
+  f[3] = b > 0 ? 1.0f : 0.0f;
+  f[4] = log_t;
+  f[5] = fe * (1.0f / 150.0f);
+  f[6] = sqrtf(fe) * (1.0f / 12.0f);
+  f[7] = fb * 0.125f * log_t;
+}
+
+// Hidden unit held by accumulator element r of tile t in lane half `half`
+// for the 32x32 MFMA output layout (row = 8 (r/4) + 4 half + r%4).
+__device__ inline int hidden_of(int t, int r, int half) { return 32 * t + 8 * (r >> 2) + 4 * half + (r & 3); }
+
+// Pre-activations of the 32 rows whose features this lane supplies:
+// lane l provides row (l % 32), features 4 (l / 32) .. +3.
+__device__ inline void hidden_tile(const uint16_t* __restrict__ params, bf16x4_t x, floatx16_t acc[4]) {
+  const int lane = lane_id();
+  const int j = lane & 31, k0 = 4 * (lane >> 5);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    bf16x4_t w;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = (short)params[kW1 + (32 * t + j) * kIn + k0 + i];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+    acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w, x, acc[t], 0, 0, 0);
+  }
+}
+
+// logit of this lane's row (valid in both halves after the exchange)
+__device__ inline float output_row(const float* b1, const float* w2, float b2, const floatx16_t acc[4]) {
+  const int half = lane_id() >> 5;
+  float s = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int h = hidden_of(t, r, half);
+      s += w2[h] * fmaxf(acc[t][r] + b1[h], 0.0f);
+    }
+  return s + __shfl_xor(s, 32, kWave) + b2;
+}
+
+__device__ inline void stage_layer_params(const uint16_t* __restrict__ params, float* b1, float* w2) {
+  for (int i = threadIdx.x; i < kHidden; i += blockDim.x) {
+    b1[i] = bf16_to_float(params[kB1 + i]);
+    w2[i] = bf16_to_float(params[kW2 + i]);
+  }
+}
+
+__device__ inline bf16x4_t load_row_features(const uint16_t* __restrict__ x, long long row, long long rows) {
+  const int k0 = 4 * (lane_id() >> 5);
+  bf16x4_t v = {0, 0, 0, 0};
+  if (row < rows) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (short)x[row * kIn + k0 + i];
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(256) mlp_forward_kernel(const uint16_t* __restrict__ params,
+                                                          const uint16_t* __restrict__ x, long long rows,
+                                                          float* __restrict__ out) {
+  __shared__ float b1[kHidden], w2[kHidden];
+  stage_layer_params(params, b1, w2);
+  __syncthreads();
+  const float b2 = bf16_to_float(params[kB2]);
+  const int lane = lane_id(), waves = blockDim.x / kWave;
+  const long long tiles = (rows + 31) / 32;
+  for (long long tile = (long long)blockIdx.x * waves + threadIdx.x / kWave; tile < tiles;
+       tile += (long long)gridDim.x * waves) {
+    const long long row = tile * 32 + (lane & 31);
+    floatx16_t acc[4];
+    hidden_tile(params, load_row_features(x, row, rows), acc);
+    const float logit = output_row(b1, w2, b2, acc);
+    if (lane < 32 && row < rows) out[row] = logit;
+  }
+}
+
+// Pair-logit BCE gradient: tuple i has rows xu[i], xv[i] and label y[i];
+// z = g(xu) + g(xv), loss = mean BCE(sigmoid(z), y).  Gradients of the mean
+// loss are added to grad[kParams] (fp32) and the summed loss to loss[0].
+__global__ void __launch_bounds__(64) mlp_grad_kernel(const uint16_t* __restrict__ params,
+                                                      const uint16_t* __restrict__ xu,
+                                                      const uint16_t* __restrict__ xv, const float* __restrict__ y,
+                                                      long long pairs, float* __restrict__ grad,
+                                                      float* __restrict__ loss) {
+  __shared__ float b1[kHidden], w2[kHidden];
+  __shared__ uint16_t dh_t[32 * 32];  // one tile of dH, [hidden 32][rows 32]
+  stage_layer_params(params, b1, w2);
+  __syncthreads();
+  const float b2 = bf16_to_float(params[kB2]);
+  const int lane = lane_id(), half = lane >> 5, j = lane & 31;
+  const float inv_pairs = 1.0f / (float)pairs;
+  floatx16_t dw1[4];  // dW1 tiles: [hidden 32][feature 32 (8 used)]
+  float dw2[4][16], db1[4][16];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      dw1[t][r] = 0.0f;
+      dw2[t][r] = 0.0f;
+      db1[t][r] = 0.0f;
+    }
+  float db2 = 0.0f, loss_sum = 0.0f;
+  const long long tiles = (pairs + 31) / 32;
+  for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const long long pair = tile * 32 + j;
+    const bool live = pair < pairs;
+    const bf16x4_t x_u = load_row_features(xu, pair, pairs);
+    const bf16x4_t x_v = load_row_features(xv, pair, pairs);
+    float z;
+    {
+      floatx16_t tmp[4];
+      hidden_tile(params, x_u, tmp);
+      z = output_row(b1, w2, b2, tmp);
+      hidden_tile(params, x_v, tmp);
+      z += output_row(b1, w2, b2, tmp) - b2;
+    }
+    // z = g(u) + g(v) counts b2 twice; the pair logit keeps one bias
+    const float label = live ? y[pair] : 0.0f;
+    const float p = 1.0f / (1.0f + expf(-z));
+    const float dz = live ? (p - label) * inv_pairs : 0.0f;
+    if (live && half == 0) loss_sum += fmaxf(z, 0.0f) - z * label + logf(1.0f + expf(-fabsf(z)));
+    if (half == 0) db2 += dz;
+    // both towers: dh = dz * w2 * [pre > 0]; accumulate dw2, db1, and dW1
+    for (int side = 0; side < 2; ++side) {
+      const bf16x4_t xs = side == 0 ? x_u : x_v;
+      floatx16_t a[4];
+      hidden_tile(params, xs, a);  // recomputed: cheaper than keeping both sides live
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int h = hidden_of(t, r, half);
+          const float pre = a[t][r] + b1[h];
+          const float act = fmaxf(pre, 0.0f);
+          const float dh = pre > 0.0f ? dz * w2[h] : 0.0f;
+          dw2[t][r] += dz * act;
+          db1[t][r] += dh;
+          // dH tile in LDS as [hidden (within tile)][row]
+          dh_t[(8 * (r >> 2) + 4 * half + (r & 3)) * 32 + j] = float_to_bf16(dh);
+        }
+        __syncthreads();
+        // dW1 tile t += dH_t (32 hidden x 32 rows) X (32 rows x 8 features):
+        // four K=8 steps over the rows; lane l supplies A[hidden l%32][rows
+        // 8s + 4 half ..+3] and B[rows 8s + 4 half ..+3][feature l%32]
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          bf16x4_t av4, bv4;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int rr = 8 * s + 4 * half + i;
+            av4[i] = (short)dh_t[j * 32 + rr];
+            // feature j of row rr: exchanged through the lanes that hold row rr
+            const int src = rr;  // lanes 0..31 hold rows 0..31 (both halves hold the row's features)
+            short fv = 0;
+            {
+              // features 0..3 live in lane rr (half 0), 4..7 in lane rr + 32
+              const int k = j;
+              const int owner = (k < 4) ? src : src + 32;
+              const int comp = k & 3;
+              short c0 = (short)__shfl((int)xs[0], owner, kWave);
+              short c1 = (short)__shfl((int)xs[1], owner, kWave);
+              short c2 = (short)__shfl((int)xs[2], owner, kWave);
+              short c3 = (short)__shfl((int)xs[3], owner, kWave);
+              const short pick = comp == 0 ? c0 : (comp == 1 ? c1 : (comp == 2 ? c2 : c3));
+              fv = k < kIn ? pick : (short)0;
+            }
+            bv4[i] = fv;
+          }
+          dw1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av4, bv4, dw1[t], 0, 0, 0);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // dW1: lane l holds dW1[hidden 32t + row(r, half)][feature l%32]
+  if (j < kIn) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) atomicAdd(&grad[kW1 + hidden_of(t, r, half) * kIn + j], dw1[t][r]);
+  }
+  // dw2, db1: sum over the 32 rows (lanes) of each half
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      float a = dw2[t][r], b = db1[t][r];
+#pragma unroll
+      for (int off = 16; off >= 1; off >>= 1) {
+        a += __shfl_xor(a, off, kWave);
+        b += __shfl_xor(b, off, kWave);
+      }
+      if (j == 0) {
+        atomicAdd(&grad[kW2 + hidden_of(t, r, half)], a);
+        atomicAdd(&grad[kB1 + hidden_of(t, r, half)], b);
+      }
+    }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    db2 += __shfl_xor(db2, off, kWave);
+    loss_sum += __shfl_xor(loss_sum, off, kWave);
+  }
+  if (lane == 0) {
+    atomicAdd(&grad[kB2], db2);
+    atomicAdd(loss, loss_sum);
+  }
+}
+
+__global__ void adam_kernel(float* __restrict__ master, uint16_t* __restrict__ params, const float* __restrict__ grad,
+                            float* __restrict__ m, float* __restrict__ v, int step, float lr, float beta1, float beta2,
+                            float eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kParams) return;
+  const float g = grad[i];
+  const float mi = beta1 * m[i] + (1.0f - beta1) * g;
+  const float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  const float mhat = mi / (1.0f - powf(beta1, (float)step));
+  const float vhat = vi / (1.0f - powf(beta2, (float)step));
+  const float w = master[i] - lr * mhat / (sqrtf(vhat) + eps);
+  master[i] = w;
+  params[i] = float_to_bf16(w);
+}
+
+// Proposal weights for every (temperature t, energy e, count b):
+// table[(t * (max_energy + 1) + e) * 64 + b] = clamp(round(4096 exp(clamp(g,
+// -6, 6))), 1, 2^21) with g the network's logit.
+__global__ void __launch_bounds__(256) weight_table_kernel(const uint16_t* __restrict__ params,
+                                                           const float* __restrict__ log_temps, int temps,
+                                                           int max_energy, uint32_t* __restrict__ table) {
+  __shared__ float b1[kHidden], w2[kHidden];
+  stage_layer_params(params, b1, w2);
+  __syncthreads();
+  const float b2 = bf16_to_float(params[kB2]);
+  const int lane = lane_id(), waves = blockDim.x / kWave;
+  const long long rows = (long long)temps * (max_energy + 1) * 64;
+  const long long tiles = rows / 32;
+  for (long long tile = (long long)blockIdx.x * waves + threadIdx.x / kWave; tile < tiles;
+       tile += (long long)gridDim.x * waves) {
+    const long long row = tile * 32 + (lane & 31);
+    const int b = (int)(row & 63);
+    const long long te = row >> 6;
+    const int e = (int)(te % (max_energy + 1));
+    const int t = (int)(te / (max_energy + 1));
+    float f[kIn];
+    point_features(b, log_temps[t], e, f);
+    const int k0 = 4 * (lane >> 5);
+    bf16x4_t x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = (short)float_to_bf16(f[k0 + i]);
+    floatx16_t acc[4];
+    hidden_tile(params, x, acc);
+    const float g = fminf(fmaxf(output_row(b1, w2, b2, acc), -6.0f), 6.0f);
+    if (lane < 32) {
+      const float w = rintf(4096.0f * expf(g));
+      table[row] = (uint32_t)fminf(fmaxf(w, 1.0f), 2097152.0f);
+    }
+  }
+}
+
+}  // namespace no3
+
+using namespace no3;
+
+extern "C" int n3_mlp_params(void) { return kParams; }
+
+extern "C" int n3_mlp_features(int count, float log_temperature, int energy, uint16_t* host_out) {
+  if (!host_out || count < 0) return set_error(N3_EINVAL, "bad feature arguments");
+  float f[kIn];
+  point_features(count, log_temperature, energy, f);
+  for (int i = 0; i < kIn; ++i) host_out[i] = float_to_bf16(f[i]);
+  return N3_OK;
+}
+
+extern "C" int n3_mlp_forward(const uint16_t* params, const uint16_t* x, int64_t rows, float* out, void* stream) {
+  if (!params || (!x && rows) || (!out && rows) || rows < 0) return set_error(N3_EINVAL, "bad forward arguments");
+  if (rows == 0) return N3_OK;
+  long long blocks = (rows + 127) / 128;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), params, x,
+                     (long long)rows, out);
+  return hip_status(hipGetLastError(), "mlp_forward_kernel launch");
+}
+
+extern "C" int n3_mlp_grad(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y,
+                           int64_t pairs, float* grad, float* loss, void* stream) {
+  if (!params || !xu || !xv || !y || !grad || !loss || pairs < 1) return set_error(N3_EINVAL, "bad grad arguments");
+  long long blocks = (pairs + 31) / 32;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3((unsigned)blocks), dim3(64), 0, as_stream(stream), params, xu, xv, y,
+                     (long long)pairs, grad, loss);
+  return hip_status(hipGetLastError(), "mlp_grad_kernel launch");
+}
+
+extern "C" int n3_mlp_adam(float* master, uint16_t* params, const float* grad, float* m, float* v, int step, float lr,
+                           float beta1, float beta2, float eps, void* stream) {
+  if (!master || !params || !grad || !m || !v || step < 1) return set_error(N3_EINVAL, "bad adam arguments");
+  hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, as_stream(stream), master, params, grad, m,
+                     v, step, lr, beta1, beta2, eps);
+  return hip_status(hipGetLastError(), "adam_kernel launch");
+}
+
+extern "C" int n3_mlp_weight_table(const uint16_t* params, const float* log_temps, int temps, int max_energy,
+                                   uint32_t* table, void* stream) {
+  if (!params || !log_temps || !table || temps < 1 || max_energy < 0)
+    return set_error(N3_EINVAL, "bad weight-table arguments");
+  const long long tiles = (long long)temps * (max_energy + 1) * 64 / 32;
+  long long blocks = (tiles + 3) / 4;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(weight_table_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), params, log_temps,
+                     temps, max_energy, table);
+  return hip_status(hipGetLastError(), "weight_table_kernel launch");
+}

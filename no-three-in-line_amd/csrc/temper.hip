@@ -28,29 +28,11 @@
 #include <vector>
 
 #include "runtime.h"
+#include "walk.h"
 #include "wave_ops.h"
 
 namespace no3 {
 
-struct WalkArgs {
-  int n;
-  int replicas;
-  int steps;
-  int bad_bias;
-  uint32_t seed_lo, seed_hi;
-  int32_t* columns;
-  int32_t* badness;
-  int32_t* scores;
-  int32_t* best_columns;
-  int32_t* best_scores;
-  int32_t* accepted;
-  const int32_t* reference;
-  int32_t* removed;
-  const uint32_t* thr;
-  int k_move;
-  const uint32_t* counters;
-  unsigned long long* proposals;
-};
 
 // One replica slot's `a.steps` proposals; `table` and `hist` are this
 // wavefront's LDS direction table and histogram.
@@ -178,6 +160,20 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     // Metropolis on the integer delta (tempering.py:91-95)
     bool accept = delta <= 0;
     if (!accept) accept = r[2] < thr[delta < a.k_move ? delta : a.k_move - 1];
+    if (a.log && lane == 0) {
+      // (badness u, badness v, energy, slot, delta, accepted): training data
+      // for the proposal network
+      const unsigned long long i = atomicAdd(a.log_count, 1ull);
+      if (i < (unsigned long long)a.log_capacity) {
+        int32_t* e = a.log + 6 * i;
+        e[0] = bad_u;
+        e[1] = bad_v;
+        e[2] = score;
+        e[3] = slot;
+        e[4] = delta;
+        e[5] = accept ? 1 : 0;
+      }
+    }
     if (!accept) continue;
 
     // badness of the unmoved slots: histograms seen from u and v now
@@ -384,7 +380,7 @@ __global__ void bump_kernel(uint32_t* counters, uint32_t d_sweep, uint32_t d_rou
   }
 }
 
-static int check_chains(const n3_chains* c) {
+int check_chains(const n3_chains* c) {
   if (!c) return set_error(N3_EINVAL, "null chains");
   if (c->n < 2 || c->n > kMaxNTable) return set_error(N3_EINVAL, "n must be in [2, %d] (got %d)", kMaxNTable, c->n);
   if (c->replicas < 1) return set_error(N3_EINVAL, "replicas must be positive");
@@ -440,10 +436,11 @@ static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan) {
   return N3_OK;
 }
 
-static WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed) {
+WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed) {
   return WalkArgs{c->n, c->replicas, steps, bad_bias, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
                   c->scores, c->best_columns, c->best_scores, c->accepted, c->reference, c->removed_points,
-                  c->move_thresholds, c->k_move, c->counters, reinterpret_cast<unsigned long long*>(c->proposals)};
+                  c->move_thresholds, c->k_move, c->counters, reinterpret_cast<unsigned long long*>(c->proposals),
+                  c->log, (long long)c->log_capacity, reinterpret_cast<unsigned long long*>(c->log_count)};
 }
 
 static int enqueue_advance(const n3_chains* c, const WalkPlan& plan, int steps, int bad_bias, uint64_t seed,
@@ -456,7 +453,7 @@ static int enqueue_advance(const n3_chains* c, const WalkPlan& plan, int steps, 
   return hip_status(hipGetLastError(), "bump_kernel launch");
 }
 
-static int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream) {
+int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream) {
   ExchangeArgs x{c->n, c->replicas, ladder, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
                  c->scores, c->removed_points, c->exchange_thresholds, c->exchange_sign, c->k_exchange, c->counters};
   const int pairs = (c->replicas / ladder) * (ladder / 2);
@@ -471,7 +468,7 @@ static int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipSt
   return hip_status(hipGetLastError(), "bump_kernel launch");
 }
 
-static int check_exchange(const n3_chains* c, int ladder) {
+int check_exchange(const n3_chains* c, int ladder) {
   if (ladder < 2 || c->replicas % ladder) return set_error(N3_EINVAL, "ladder_size must divide replica count");
   if (!c->exchange_thresholds || !c->exchange_sign || c->k_exchange < 2)
     return set_error(N3_EINVAL, "exchange thresholds missing");

@@ -1,0 +1,14 @@
+// Tempering helpers shared between the uniform and the learned walks.
+#pragma once
+
+#include "runtime.h"
+#include "walk.h"
+
+namespace no3 {
+
+int check_chains(const n3_chains* c);
+int check_exchange(const n3_chains* c, int ladder);
+WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed);
+int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream);
+
+}  // namespace no3

@@ -1,0 +1,31 @@
+// Arguments of a tempering walk launch (one replica slot per wavefront).
+#pragma once
+
+#include <stdint.h>
+
+namespace no3 {
+
+struct WalkArgs {
+  int n;
+  int replicas;
+  int steps;
+  int bad_bias;
+  uint32_t seed_lo, seed_hi;
+  int32_t* columns;
+  int32_t* badness;
+  int32_t* scores;
+  int32_t* best_columns;
+  int32_t* best_scores;
+  int32_t* accepted;
+  const int32_t* reference;
+  int32_t* removed;
+  const uint32_t* thr;
+  int k_move;
+  const uint32_t* counters;
+  unsigned long long* proposals;
+  int32_t* log;                   // optional proposal log, 6 int32 per entry
+  long long log_capacity;
+  unsigned long long* log_count;
+};
+
+}  // namespace no3

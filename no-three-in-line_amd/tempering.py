@@ -104,6 +104,8 @@ class TemperingChains:
         # replicas (wavefronts) per workgroup of the walk; 0 lets the library
         # choose (see autotune.py for the measured choice per n)
         self.block_waves = 0
+        # optional device log of uniform-walk proposals (learned.ProposalLog)
+        self.log = None
 
     # -- helpers -----------------------------------------------------------
     def _rescore(self, columns: torch.Tensor, scores: torch.Tensor, badness: torch.Tensor,
@@ -145,6 +147,10 @@ class TemperingChains:
         c.counters = self.counters.data_ptr()
         c.proposals = self.proposals.data_ptr()
         c.block_waves = int(self.block_waves)
+        if self.log is not None:
+            c.log = self.log.entries.data_ptr()
+            c.log_capacity = self.log.capacity
+            c.log_count = self.log.count.data_ptr()
         return c
 
     def _check_umbrella(self) -> None:

@@ -16,6 +16,9 @@
  *   n3o_evaluate_two_switches    accelerated.py:122-195
  *   n3o_walk (tempering replay)  tempering.py:48-110    (walk kernel step semantics)
  *   n3o_exchange                 tempering.py:169-191   (adjacent-temperature swaps)
+ *   n3o_advance_learned          the walk above with learned first-slot proposals
+ *                                and the Metropolis-Hastings test (no reference
+ *                                counterpart; the rule is stated at the function)
  *
  * Deliberate, documented differences from the reference (the same in the
  * product and in this oracle, so parity is exact):
@@ -78,6 +81,7 @@ static inline uint32_t mulhi32(uint32_t a, uint32_t b)
 #define TAG_MOVE 0x4D4F5645u /* "MOVE" */
 #define TAG_BIAS 0x42494153u /* "BIAS" */
 #define TAG_EXCH 0x45584348u /* "EXCH" */
+#define TAG_LEARN 0x4C524E44u /* "LRND" */
 
 /* ------------------------------------------------------------------------ */
 /* Seeded two-permutation state generator                                   */
@@ -597,4 +601,163 @@ int n3o_exchange(int n, int replicas, int ladder, uint64_t seed, uint32_t round,
         }
     }
     return exchanged;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Learned-proposal walk replay                                               */
+/* ------------------------------------------------------------------------ */
+
+/* thr64[r][d + k] = floor(2^32 exp(-d / T_r)) saturating at 2^63, d in [-k, k). */
+void n3o_metropolis_thresholds64(const float* temperatures, int replicas, int k, uint64_t* thr)
+{
+    const double cap = 9223372036854775808.0;
+    for (int r = 0; r < replicas; ++r) {
+        const double t = (double)temperatures[r];
+        for (int d = -k; d < k; ++d) {
+            const double x = ldexp(exp(-(double)d / t), 32);
+            thr[(int64_t)r * 2 * k + (d + k)] = x >= cap ? ((uint64_t)1 << 63) : (uint64_t)floor(x);
+        }
+    }
+}
+
+/* The walk of walk_slot with the first slot drawn from
+ *   pi(k) = 0.2 / m + 0.8 w(k) / S1   (w(k) = wt[E][min(bad[k], 63)])
+ * by a 32-bit coin (< floor(0.2 * 2^32): uniform u = mulhi(r1, m); else the
+ * first k whose inclusive prefix of w exceeds floor(r1 * S1 / 2^32)), the
+ * second uniformly from the other m - 1 (v' = mulhi(r2, m - 1), v = v' + [v'
+ * >= u]), and the Metropolis-Hastings test
+ *   r3 * N * S1' < thr64(delta) * N' * S1,  N = 2 S1 + 4 m (w(u) + w(v))
+ * in 128-bit integers, primes for the moved state.  Philox counter
+ * (step, sweep, slot, "LRND"). */
+static int64_t walk_slot_learned(int n, int steps, uint64_t seed, uint32_t sweep, uint32_t slot, int32_t* col,
+                                 int32_t* bad, int32_t* score_io, int32_t* best_col, int32_t* best_score,
+                                 int32_t* accepted, const int32_t* reference, int32_t* removed_io, const uint32_t* wt,
+                                 int max_energy, const uint64_t* thr64, int k64)
+{
+    const int m = 2 * n;
+    int32_t score = *score_io, removed = *removed_io;
+    int32_t db[N3O_MAXM], nb[N3O_MAXM];
+    uint32_t w[N3O_MAXM], w2[N3O_MAXM];
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int64_t valid_count = 0;
+#define ROW(e) ((int64_t)((e) < 0 ? 0 : ((e) > max_energy ? max_energy : (e))) * 64)
+    uint32_t s1 = 0;
+    for (int i = 0; i < m; ++i) {
+        w[i] = wt[ROW(score) + (bad[i] < 63 ? bad[i] : 63)];
+        s1 += w[i];
+    }
+    for (int step = 0; step < steps; ++step) {
+        uint32_t ctr[4] = {(uint32_t)step, sweep, slot, TAG_LEARN}, r[4];
+        n3o_philox4x32_10(ctr, key, r);
+        int u;
+        if (r[0] < 858993459u) {
+            u = (int)mulhi32(r[1], (uint32_t)m);
+        } else {
+            const uint32_t target = (uint32_t)(((uint64_t)r[1] * (uint64_t)s1) >> 32);
+            uint32_t acc = 0;
+            u = m - 1;
+            for (int i = 0; i < m; ++i) {
+                acc += w[i];
+                if (acc > target) {
+                    u = i;
+                    break;
+                }
+            }
+        }
+        const int vv = (int)mulhi32(r[2], (uint32_t)(m - 1));
+        const int v = vv + (vv >= u ? 1 : 0);
+        const int32_t cu = col[u], cv = col[v];
+        const int valid = score > 0 && (u >> 1) != (v >> 1) && cu != cv && col[u ^ 1] != cv && col[v ^ 1] != cu;
+        const int ur = (u >> 1) * 2, vr = (v >> 1) * 2;
+        const int dd = (cu == reference[ur] || cu == reference[ur + 1]) + (cv == reference[vr] || cv == reference[vr + 1]) -
+                       (cv == reference[ur] || cv == reference[ur + 1]) - (cu == reference[vr] || cu == reference[vr + 1]);
+        if (!valid) continue;
+        valid_count += 1;
+        for (int i = 0; i < m; ++i) db[i] = 0;
+        int32_t delta = 0;
+        const int ru = u >> 1, rv = v >> 1;
+        for (int i = 0; i < m - 1; ++i) {
+            if (i == u || i == v) continue;
+            const int ri = i >> 1, ci = col[i];
+            for (int j = i + 1; j < m; ++j) {
+                if (j == u || j == v) continue;
+                const int rj = j >> 1, cj = col[j];
+                int d = det0(ru, cv, ri, ci, rj, cj) - det0(ru, cu, ri, ci, rj, cj);
+                if (d) {
+                    delta += d;
+                    db[u] += d;
+                    db[i] += d;
+                    db[j] += d;
+                }
+                d = det0(rv, cu, ri, ci, rj, cj) - det0(rv, cv, ri, ci, rj, cj);
+                if (d) {
+                    delta += d;
+                    db[v] += d;
+                    db[i] += d;
+                    db[j] += d;
+                }
+            }
+        }
+        for (int kk = 0; kk < m; ++kk) {
+            if (kk == u || kk == v) continue;
+            const int d = det0(ru, cv, rv, cu, kk >> 1, col[kk]) - det0(ru, cu, rv, cv, kk >> 1, col[kk]);
+            if (d) {
+                delta += d;
+                db[u] += d;
+                db[v] += d;
+                db[kk] += d;
+            }
+        }
+        uint32_t s1n = 0;
+        for (int i = 0; i < m; ++i) {
+            nb[i] = bad[i] + db[i];
+            w2[i] = wt[ROW(score + delta) + (nb[i] < 63 ? nb[i] : 63)];
+            s1n += w2[i];
+        }
+        const uint64_t big_n = 2ull * s1 + 4ull * (uint64_t)m * ((uint64_t)w[u] + w[v]);
+        const uint64_t big_n2 = 2ull * s1n + 4ull * (uint64_t)m * ((uint64_t)w2[u] + w2[v]);
+        const int d = delta < -k64 ? -k64 : (delta > k64 - 1 ? k64 - 1 : delta);
+        const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * s1n;
+        const unsigned __int128 rhs = (unsigned __int128)thr64[d + k64] * big_n2 * s1;
+        if (!(lhs < rhs)) continue;
+        col[u] = cv;
+        col[v] = cu;
+        score += delta;
+        removed += dd;
+        *accepted += 1;
+        for (int i = 0; i < m; ++i) {
+            bad[i] = nb[i];
+            w[i] = w2[i];
+        }
+        s1 = s1n;
+        if (score < *best_score) {
+            *best_score = score;
+            for (int i = 0; i < m; ++i) best_col[i] = col[i];
+        }
+    }
+#undef ROW
+    *score_io = score;
+    *removed_io = removed;
+    return valid_count;
+}
+
+int64_t n3o_advance_learned(int n, int replicas, int steps, uint64_t seed, uint32_t sweep, int32_t* columns,
+                            int32_t* badness, int32_t* scores, int32_t* best_columns, int32_t* best_scores,
+                            int32_t* accepted, const int32_t* reference, int32_t* removed, const uint32_t* weights,
+                            const int32_t* temp_index, int max_energy, const uint64_t* thr64, int k64, int threads)
+{
+    const int m = 2 * n;
+    int64_t valid = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : valid)
+    for (int r = 0; r < replicas; ++r) {
+        valid += walk_slot_learned(n, steps, seed, sweep, (uint32_t)r, columns + (int64_t)r * m,
+                                   badness + (int64_t)r * m, scores + r, best_columns + (int64_t)r * m,
+                                   best_scores + r, accepted + r, reference, removed + r,
+                                   weights + (int64_t)temp_index[r] * (max_energy + 1) * 64, max_energy,
+                                   thr64 + (int64_t)r * 2 * k64, k64);
+    }
+    return valid;
 }

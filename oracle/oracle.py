@@ -66,6 +66,14 @@ def lib():
             _i32p, _i32p, _i32p, _i32p, _u32p, _i8p, ctypes.c_int,
         ]
         L.n3o_exchange.restype = ctypes.c_int
+        _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+        L.n3o_metropolis_thresholds64.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _u64p]
+        L.n3o_advance_learned.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+            _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _u32p, _i32p, ctypes.c_int, _u64p,
+            ctypes.c_int, ctypes.c_int,
+        ]
+        L.n3o_advance_learned.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -194,6 +202,18 @@ class TemperingReplay:
         self.sweep += 1
         return int(valid)
 
+    def advance_learned(self, steps: int, weights, temp_index, max_energy: int, thr64, threads: int = 0) -> int:
+        """The learned-proposal walk (see n3o_advance_learned) on the same state."""
+        weights = np.ascontiguousarray(weights, dtype=np.uint32)
+        thr64 = np.ascontiguousarray(thr64, dtype=np.uint64)
+        k64 = thr64.shape[1] // 2
+        valid = lib().n3o_advance_learned(
+            self.n, self.replicas, steps, self.seed, self.sweep, self.columns, self.badness, self.scores,
+            self.best_columns, self.best_scores, self.accepted, self.reference, self.removed_points,
+            weights.reshape(-1), np.ascontiguousarray(temp_index, dtype=np.int32), max_energy, thr64, k64, threads)
+        self.sweep += 1
+        return int(valid)
+
     def exchange(self, ladder_size: int) -> int:
         thr, sign = exchange_thresholds(self.temperatures, ladder_size, self.k_exchange)
         count = lib().n3o_exchange(self.n, self.replicas, ladder_size, self.seed, self.round, self.columns,
@@ -204,3 +224,56 @@ class TemperingReplay:
 
 def cpu_count() -> int:
     return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def metropolis_thresholds64(temperatures, k: int) -> np.ndarray:
+    t = np.ascontiguousarray(temperatures, dtype=np.float32)
+    thr = np.empty((len(t), 2 * k), dtype=np.uint64)
+    lib().n3o_metropolis_thresholds64(t, len(t), k, thr)
+    return thr
+
+
+# -- proposal network in float64 (the check of the MFMA kernels) ---------------
+
+MLP_IN, MLP_HIDDEN = 8, 128
+
+
+def bf16_to_f64(h) -> np.ndarray:
+    return (np.asarray(h, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def mlp_unpack(params_bf16):
+    p = bf16_to_f64(params_bf16)
+    w1 = p[: MLP_HIDDEN * MLP_IN].reshape(MLP_HIDDEN, MLP_IN)
+    b1 = p[MLP_HIDDEN * MLP_IN: MLP_HIDDEN * MLP_IN + MLP_HIDDEN]
+    w2 = p[MLP_HIDDEN * MLP_IN + MLP_HIDDEN: MLP_HIDDEN * MLP_IN + 2 * MLP_HIDDEN]
+    b2 = p[-1]
+    return w1, b1, w2, b2
+
+
+def mlp_forward(params_bf16, x_bf16) -> np.ndarray:
+    """g(x) = w2 . relu(W1 x + b1) + b2 in float64 on bf16-valued inputs."""
+    w1, b1, w2, b2 = mlp_unpack(params_bf16)
+    x = bf16_to_f64(x_bf16).reshape(-1, MLP_IN)
+    return np.maximum(x @ w1.T + b1, 0.0) @ w2 + b2
+
+
+def mlp_pair_grad(params_bf16, xu_bf16, xv_bf16, y) -> tuple[np.ndarray, float]:
+    """Gradient (float64, packed like the parameters) and summed loss of the
+    mean BCE(sigmoid(g(xu) + g(xv) - b2), y) over the pairs."""
+    w1, b1, w2, b2 = mlp_unpack(params_bf16)
+    xu = bf16_to_f64(xu_bf16).reshape(-1, MLP_IN)
+    xv = bf16_to_f64(xv_bf16).reshape(-1, MLP_IN)
+    y = np.asarray(y, dtype=np.float64)
+    pre_u, pre_v = xu @ w1.T + b1, xv @ w1.T + b1
+    h_u, h_v = np.maximum(pre_u, 0), np.maximum(pre_v, 0)
+    z = h_u @ w2 + h_v @ w2 + b2
+    loss = float(np.sum(np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z)))))
+    dz = (1.0 / (1.0 + np.exp(-z)) - y) / len(y)
+    dh_u = np.outer(dz, w2) * (pre_u > 0)
+    dh_v = np.outer(dz, w2) * (pre_v > 0)
+    g_w1 = dh_u.T @ xu + dh_v.T @ xv
+    g_b1 = dh_u.sum(0) + dh_v.sum(0)
+    g_w2 = h_u.T @ dz + h_v.T @ dz
+    g_b2 = dz.sum()
+    return np.concatenate([g_w1.reshape(-1), g_b1, g_w2, [g_b2]]), loss

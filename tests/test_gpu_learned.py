@@ -1,0 +1,152 @@
+"""Proposal network kernels against the float64 oracle, and the learned walk
+against its CPU replay (bit-exact)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(rng, scale=1.0):
+    w1 = rng.normal(0, scale * math.sqrt(2 / 8), (128, 8))
+    b1 = rng.normal(0, 0.1, 128)
+    w2 = rng.normal(0, scale * math.sqrt(1 / 128), 128)
+    return np.concatenate([w1.reshape(-1), b1, w2, [0.3]]).astype(np.float32)
+
+
+def _net(pkg, dev, rng):
+    net = pkg.learned.ProposalNetwork(seed=0, device=dev)
+    master = _params(rng)
+    net.master.copy_(torch.from_numpy(master))
+    net.params.copy_(torch.from_numpy(pkg.learned.to_bf16(master).view(np.int16)))
+    return net
+
+
+def _features(pkg, rng, rows):
+    f = pkg.learned.point_features(rng.integers(0, 40, rows), np.log(rng.uniform(0.05, 2.0, rows)),
+                                   rng.integers(0, 600, rows))
+    return pkg.learned.to_bf16(f)
+
+
+def test_forward_matches_float64(pkg, oracle, dev):
+    rng = np.random.default_rng(1)
+    net = _net(pkg, dev, rng)
+    for rows in (1, 31, 32, 33, 1000, 4099):
+        x = _features(pkg, rng, rows)
+        got = net.forward(torch.from_numpy(x.view(np.int16)).to(dev)).cpu().numpy()
+        ref = oracle.mlp_forward(net.params.cpu().numpy().view(np.uint16), x)
+        # bf16 inputs and weights on both sides; the kernel accumulates in fp32
+        np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-4)
+
+
+def test_oracle_gradient_is_the_loss_derivative(oracle):
+    """Finite differences of the float64 loss against mlp_pair_grad."""
+    rng = np.random.default_rng(2)
+    p = _params(rng)
+    xu = rng.normal(0, 1, (50, 8)).astype(np.float32)
+    xv = rng.normal(0, 1, (50, 8)).astype(np.float32)
+    y = (rng.random(50) < 0.4).astype(np.float64)
+    from no_three_in_line_amd.learned import to_bf16
+
+    pb, xub, xvb = to_bf16(p), to_bf16(xu), to_bf16(xv)
+    g, _ = oracle.mlp_pair_grad(pb, xub, xvb, y)
+    base = oracle.bf16_to_f64(pb)
+
+    def loss_at(vec):
+        w1 = vec[:1024].reshape(128, 8)
+        b1, w2, b2 = vec[1024:1152], vec[1152:1280], vec[1280]
+        a = oracle.bf16_to_f64(xub).reshape(-1, 8)
+        b = oracle.bf16_to_f64(xvb).reshape(-1, 8)
+        z = np.maximum(a @ w1.T + b1, 0) @ w2 + np.maximum(b @ w1.T + b1, 0) @ w2 + b2
+        return np.mean(np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
+
+    for i in list(rng.choice(1024, 12, replace=False)) + [1030, 1100, 1160, 1279, 1280]:
+        e = np.zeros_like(base)
+        e[i] = 1e-6
+        fd = (loss_at(base + e) - loss_at(base - e)) / 2e-6
+        assert abs(fd - g[i]) < 1e-6 + 1e-4 * abs(g[i])
+
+
+def test_gradient_matches_float64(pkg, oracle, dev):
+    rng = np.random.default_rng(3)
+    net = _net(pkg, dev, rng)
+    for pairs in (7, 64, 1000):
+        xu, xv = _features(pkg, rng, pairs), _features(pkg, rng, pairs)
+        y = (rng.random(pairs) < 0.3).astype(np.float32)
+        grad, loss = net.gradient(torch.from_numpy(xu.view(np.int16)).to(dev),
+                                  torch.from_numpy(xv.view(np.int16)).to(dev), torch.from_numpy(y).to(dev))
+        ref, ref_loss = oracle.mlp_pair_grad(net.params.cpu().numpy().view(np.uint16), xu, xv, y)
+        got = grad.cpu().numpy()
+        # dH is rounded to bf16 before the dW1 MFMA (8 significant bits):
+        # tolerance 2% of the gradient's scale per block of parameters
+        for lo, hi in ((0, 1024), (1024, 1152), (1152, 1280), (1280, 1281)):
+            scale = np.abs(ref[lo:hi]).max() + 1e-12
+            assert np.abs(got[lo:hi] - ref[lo:hi]).max() <= 2e-2 * scale, (pairs, lo)
+        assert abs(loss - ref_loss / pairs) < 1e-4 * max(1.0, abs(ref_loss / pairs))
+
+
+def test_weight_table(pkg, oracle, dev):
+    rng = np.random.default_rng(4)
+    net = _net(pkg, dev, rng)
+    log_temps = np.log(np.array([0.05, 0.3, 2.0], np.float32))
+    table = net.weight_table(log_temps, max_energy=40).cpu().numpy().view(np.uint32)
+    assert table.shape == (3, 41, 64) and table.min() >= 1 and table.max() <= 1 << 21
+    L = pkg.learned
+    for t, lt in enumerate(log_temps):
+        for e in (0, 7, 40):
+            x = L.to_bf16(L.point_features(np.arange(64), lt, e))
+            g = np.clip(oracle.mlp_forward(net.params.cpu().numpy().view(np.uint16), x), -6, 6)
+            w = np.clip(np.rint(4096 * np.exp(g)), 1, 1 << 21)
+            # the device computes the features with its own sqrt/log before
+            # bf16 rounding: allow a few bf16 ulps of the logit
+            np.testing.assert_allclose(table[t, e], w, rtol=2e-2)
+
+
+def _cols(oracle, n, replicas, seed):
+    p, q = oracle.generate_states(seed, n, replicas)
+    return np.stack([np.column_stack((p[i], q[i])).reshape(-1) for i in range(replicas)]).astype(np.int32)
+
+
+@pytest.mark.parametrize("n,replicas,ladder,sweeps", [(12, 32, 16, 8), (33, 64, 32, 4), (75, 64, 64, 3)])
+def test_learned_walk_matches_replay(pkg, oracle, dev, n, replicas, ladder, sweeps):
+    rng = np.random.default_rng(n)
+    net = _net(pkg, dev, rng)
+    cols = _cols(oracle, n, replicas, seed=n + 1)
+    temps = np.tile(np.geomspace(0.05, 2.0, ladder), replicas // ladder).astype(np.float32)
+    chains = pkg.TemperingChains(cols, temps, seed=77)
+    props = pkg.learned.prepare(chains, net)
+    replay = oracle.TemperingReplay(cols, temps, 77, k_move=chains.k_move, k_exchange=chains._exchange_table(ladder)[2])
+    thr64 = props.thr64.cpu().numpy().view(np.uint64)
+    assert np.array_equal(thr64, oracle.metropolis_thresholds64(temps, chains.k_move))
+    weights = props.weights.cpu().numpy().view(np.uint32)
+    index = props.temp_index.cpu().numpy()
+    for _ in range(sweeps):
+        pkg.learned.run_learned(chains, props, 1, 2 * n, ladder)
+        replay.advance_learned(2 * n, weights, index, props.max_energy, thr64)
+        replay.exchange(ladder)
+        torch.cuda.synchronize()
+        for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted", "removed_points"):
+            assert np.array_equal(getattr(chains, name).cpu().numpy(), getattr(replay, name)), name
+    chains.validate()
+
+
+def test_training_on_logged_proposals(pkg, dev):
+    chains = pkg.seeded_chains(40, 512, 64, seed=5)
+    log = pkg.learned.ProposalLog(1 << 18, dev)
+    chains.log = log
+    chains.run(30, 80, 64)
+    torch.cuda.synchronize()
+    rows = log.rows()
+    assert rows.shape[0] == min(int(log.count.item()), log.capacity) > 10000
+    r = rows.cpu().numpy()
+    assert set(np.unique(r[:, 5])) <= {0, 1} and np.all(r[r[:, 4] <= 0, 5] == 1)
+    net = pkg.learned.ProposalNetwork(seed=1, device=dev)
+    xu, xv, y = pkg.learned.training_tensors(rows, chains.temperatures_host, dev)
+    losses = net.train(xu, xv, y, epochs=4)
+    assert losses[-1] < losses[0]
+    props = pkg.learned.prepare(chains, net)
+    pkg.learned.run_learned(chains, props, 20, 80, 64)
+    torch.cuda.synchronize()
+    chains.validate()

@@ -113,6 +113,52 @@ def tempering_rate(pkg, torch, sweeps: int = 100) -> dict:
             "best_E": int(chains.best_scores.min().item())}
 
 
+def batched_rate(pkg, torch, sweeps: int = 1000) -> dict:
+    """Mixed sizes n in {40, 52, 61, 75, 83, 100} with {8192, 4096, 4096,
+    2048, 2048, 1024} replicas (ladders of 64, geometric 0.05..2.0), 2n
+    proposals per sweep, one walk launch per sweep for all sizes, graph-
+    captured; replicas per workgroup from the autotuner."""
+    sizes = {40: 8192, 52: 4096, 61: 4096, 75: 2048, 83: 2048, 100: 1024}
+    t0 = time.perf_counter()
+    shapes = pkg.autotune.tune(sizes, sweeps=2, candidates=(2, 4, 8, 16))
+    tune_seconds = time.perf_counter() - t0
+    groups = []
+    for i, (n, replicas) in enumerate(sizes.items()):
+        chains = pkg.seeded_chains(n, replicas, 64, seed=40 + i)
+        chains.block_waves = shapes[n]
+        groups.append(pkg.BatchGroup(chains, 64, 2 * n))
+    pkg.run_batched(groups, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pkg.run_batched(groups, sweeps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    proposals = sum(g.chains.temperatures_host.size * g.steps for g in groups) * sweeps
+    for g in groups:
+        g.chains.validate(range(0, len(g.chains.temperatures_host), 211))
+    return {"sizes": {str(n): r for n, r in sizes.items()}, "sweeps": sweeps, "seconds": elapsed,
+            "proposals_per_s": proposals / elapsed, "block_waves": {str(n): w for n, w in shapes.items()},
+            "autotune_seconds": tune_seconds,
+            "best_E": {str(g.chains.n): int(g.chains.best_scores.min().item()) for g in groups}}
+
+
+def learned_ab(pkg, seconds: float) -> dict:
+    """n=75, 4096 replicas as 64 ladders x 64 temperatures: uniform against
+    learned proposals, same seeds, replicas and GPU, equal wall-clock budget
+    (the learned side pays for logging and training inside its budget)."""
+    res = pkg.learned.ab_run(75, 4096, 64, seconds=seconds, seed=11, state_seed=1, train_sweeps=300, chunk=100)
+    out = {"budget_seconds_each": seconds}
+    for side in ("uniform", "learned"):
+        r = res[side]
+        out[side] = {"sweeps": r["sweeps"], "best_E": r["trace"][-1][1],
+                     "seconds_to_E": r["threshold_seconds"]}
+    out["learned"]["training_tuples"] = res["learned"]["training_tuples"]
+    out["learned"]["seconds_before_learned_proposals"] = res["learned"]["seconds_until_learned_proposals"]
+    out["time_to_E12"] = {"uniform": res["uniform"]["threshold_seconds"]["12"],
+                          "learned": res["learned"]["threshold_seconds"]["12"]}
+    return out
+
+
 def calibrate_copy() -> int:
     """A plain device-to-device copy of 256 MiB, five times: profiled with
     FETCH_SIZE / WRITE_SIZE it gives the counters' reading for known bytes."""
@@ -133,7 +179,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-temper", action="store_true", help="skip the tempering rate")
+    ap.add_argument("--no-temper", action="store_true", help="skip the tempering, batched and learned legs")
+    ap.add_argument("--ab-seconds", type=float, default=15.0, help="wall-clock budget of each A/B side")
     ap.add_argument("--method", type=int, default=0, help="0 auto (direction histograms), 1 determinant")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
@@ -197,6 +244,10 @@ def main() -> int:
     if not args.no_temper:
         log("bench: tempering rate")
         extra["tempering_n75_64x64"] = tempering_rate(pkg, torch)
+        log("bench: batched uneven sizes")
+        extra["batched_uneven"] = batched_rate(pkg, torch)
+        log("bench: learned vs uniform proposals")
+        extra["learned_vs_uniform"] = learned_ab(pkg, args.ab_seconds)
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline")
@@ -212,6 +263,8 @@ def main() -> int:
                    "n": N, "population": POPULATION, "seed": SEED,
                    "method": "direction-histogram" if args.method != 1 else "determinant"},
         "parity_subsample_exact": exact,
+        "device": {"name": torch.cuda.get_device_name(dev),
+                   "compute_units": torch.cuda.get_device_properties(dev).multi_processor_count},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extra,

@@ -86,6 +86,15 @@ int n3_evaluate_two_switches(const int32_t* p, const int32_t* q, const int64_t* 
 int n3_evaluate_transpositions(const int32_t* p, const int32_t* q, const int64_t* badness, int n, int layer,
                                int row, int64_t current_score, int64_t* scores, void* stream);
 
+/* accelerated.evaluate_three_cycles (accelerated.py:198-276) on the device:
+ * int64 scores[count] for candidates int32[count][4] = (u, v, w,
+ * orientation), orientation 0 moving u <- col v, v <- col w, w <- col u and
+ * 1 moving u <- col w, w <- col v, v <- col u (search.py:38-65 lists the
+ * legal ones); badness is int64[2n]. */
+int n3_evaluate_three_cycles(const int32_t* p, const int32_t* q, const int64_t* badness, int n,
+                             const int32_t* candidates, int64_t count, int64_t current_score, int64_t* scores,
+                             void* stream);
+
 /* Full 2-switch table of a population: table[s][u][v] is
  * evaluate_two_switches(state s, u, scores[s], badness[s])[v] as int32 with
  * the sentinel 1<<30 (the all-anchor sweep of analysis.py:57-74).  `scores`

@@ -18,10 +18,12 @@ from .gpu import (
 from .gpu_delta import (
     delta_table,
     delta_table_device,
+    evaluate_three_cycles,
     evaluate_transpositions,
     evaluate_two_switches,
     score_two_switches_gpu,
 )
+from .moves import apply_three_cycle, legal_three_cycle_candidates
 from .state import PairPermutations, points_to_permutations, random_state
 from .tempering import TemperingChains
 from .batched import BatchGroup, run_batched, seeded_chains
@@ -30,7 +32,8 @@ from . import autotune, learned
 __all__ = [
     "SCORE_AUTO", "SCORE_DETERMINANT", "SCORE_SLOPE", "SENTINEL32", "SENTINEL64",
     "check_status", "generate_states", "score_population_badness", "score_population_device", "score_population_gpu",
-    "delta_table", "delta_table_device", "evaluate_transpositions", "evaluate_two_switches", "score_two_switches_gpu",
+    "delta_table", "delta_table_device", "evaluate_three_cycles", "evaluate_transpositions",
+    "apply_three_cycle", "legal_three_cycle_candidates", "evaluate_two_switches", "score_two_switches_gpu",
     "PairPermutations", "points_to_permutations", "random_state", "TemperingChains", "_lib",
     "BatchGroup", "run_batched", "seeded_chains", "autotune", "learned",
 ]

@@ -154,6 +154,136 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
   }
 }
 
+// 3-cycles (accelerated.py:198-276): candidate (u, v, w, orientation) moves
+// u <- col v, v <- col w, w <- col u (orientation 0) or u <- col w,
+// w <- col v, v <- col u (1).  The reference's three pair loops count the
+// moved state's triples through u', v' or w' by inclusion-exclusion; here
+//   new = bad'(u') + bad'(v') + bad'(w') - j'(u'v') - j'(u'w') - j'(v'w') + t'
+// from three direction histograms (j' the points on the line through two
+// moved points, t' whether the three are collinear), and the old side
+//   bad[u] + bad[v] + bad[w] - j(uv) - j(uw) - j(vw) + t
+// uses the caller's badness like the reference.
+template <int CH>
+__global__ void __launch_bounds__(256) three_cycle_kernel(const int32_t* __restrict__ p,
+                                                          const int32_t* __restrict__ q, int n,
+                                                          const int32_t* __restrict__ candidates, long long count,
+                                                          long long current, const long long* __restrict__ badness,
+                                                          long long* __restrict__ out, int* __restrict__ status,
+                                                          const uint16_t* __restrict__ table_g, DirTableInfo info) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  stage_table(table_g, table, info.entries);
+  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+  const int m = 2 * n, width = info.width, offset = info.offset, zero_id = info.directions;
+  int key[CH];
+  int outside = 0;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    key[c] = 0;
+    if (k < m) {
+      const int col = (k & 1) ? q[k >> 1] : p[k >> 1];
+      outside |= (col < 0 || col >= n);
+      key[c] = (k >> 1) * width + col;
+    }
+  }
+  if (__any(outside)) {
+    if (lane == 0) atomicOr(status, 1);
+    return;
+  }
+  const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+    if (lane + kWave * c >= m) key[c] = key0;
+  const int trash = trash_word(info, lane);
+  for (long long ci = (long long)blockIdx.x * waves + wave; ci < count; ci += (long long)gridDim.x * waves) {
+    const int u = candidates[4 * ci], v = candidates[4 * ci + 1], w = candidates[4 * ci + 2];
+    const int orientation = candidates[4 * ci + 3];
+    if (u < 0 || u >= m || v < 0 || v >= m || w < 0 || w >= m) {
+      if (lane == 0) atomicOr(status, 2);
+      continue;
+    }
+    const int key_u = slot_value<CH>(key, u), key_v = slot_value<CH>(key, v), key_w = slot_value<CH>(key, w);
+    const int ru = u >> 1, rv = v >> 1, rw = w >> 1;
+    const int cu = key_u - ru * width, cv = key_v - rv * width, cw = key_w - rw * width;
+    // old side
+    const int id_uv = table[key_v - key_u + offset], id_uw = table[key_w - key_u + offset];
+    const int id_vw = table[key_w - key_v + offset];
+    int joint = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      const bool live = k < m;
+      const int a = table[key[c] - key_u + offset];
+      const int b = table[key[c] - key_v + offset];
+      joint += __popcll(__ballot(live && k != u && k != v && (a == id_uv || a == zero_id)));
+      joint += __popcll(__ballot(live && k != u && k != w && (a == id_uw || a == zero_id)));
+      joint += __popcll(__ballot(live && k != v && k != w && (b == id_vw || b == zero_id)));
+    }
+    const int triple = (key_v != key_u && key_w != key_u && id_uv == id_uw) ? 1 : 0;
+    const long long old_affected = badness[u] + badness[v] + badness[w] - joint + triple;
+    // moved state
+    int nu, nv, nw;
+    if (orientation == 0) {
+      nu = cv;
+      nv = cw;
+      nw = cu;
+    } else {
+      nu = cw;
+      nw = cv;
+      nv = cu;
+    }
+    const int ku = ru * width + nu, kv = rv * width + nv, kw = rw * width + nw;
+    const int id2_uv = table[kv - ku + offset], id2_uw = table[kw - ku + offset], id2_vw = table[kw - kv + offset];
+    int keys[CH], cnt[CH], id[CH];
+    bool include[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      keys[c] = (k == u) ? ku : ((k == v) ? kv : ((k == w) ? kw : key[c]));
+    }
+    uint32_t part_uv = 0, part_w = 0;
+    int joint_new = 0;
+    // from u'
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
+    direction_counts<CH>(table, hist, trash, offset, ku, keys, include, cnt, id);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      if (include[c]) part_uv += (uint32_t)(cnt[c] - 1);
+      joint_new += __popcll(__ballot(include[c] && k != v && id[c] == id2_uv));
+      joint_new += __popcll(__ballot(include[c] && k != w && id[c] == id2_uw));
+    }
+    // from v'
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+    direction_counts<CH>(table, hist, trash, offset, kv, keys, include, cnt, id);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      if (include[c]) part_uv += (uint32_t)(cnt[c] - 1) << 16;
+      joint_new += __popcll(__ballot(include[c] && k != w && id[c] == id2_vw));
+    }
+    // from w'
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != w;
+    direction_counts<CH>(table, hist, trash, offset, kw, keys, include, cnt, id);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (include[c]) part_w += (uint32_t)(cnt[c] - 1);
+    const uint32_t packed = wave_sum_dpp(part_uv), total_w = wave_sum_dpp(part_w);
+    const int triple_new = (id2_uv == id2_uw) ? 1 : 0;
+    const long long new_affected = (long long)((packed & 0xFFFFu) >> 1) + (long long)((packed >> 16) >> 1) +
+                                   (long long)(total_w >> 1) - joint_new + triple_new;
+    if (lane == 0) out[ci] = current - old_affected + new_affected;
+  }
+}
+
 template <int CH, typename BT, typename OT>
 static int launch_switch(const SwitchArgs& args, hipStream_t stream) {
   const uint16_t* table = nullptr;
@@ -274,4 +404,44 @@ extern "C" int n3_delta_table(const int32_t* p, const int32_t* q, const int32_t*
   if (count == 0) return N3_OK;
   SwitchArgs a{p, q, n, (long long)count, -1, -1, 1, badness, scores, 0, table, 2 * n, nullptr};
   return run_switch(a, false, as_stream(stream));
+}
+
+extern "C" int n3_evaluate_three_cycles(const int32_t* p, const int32_t* q, const int64_t* badness, int n,
+                                        const int32_t* candidates, int64_t count, int64_t current_score,
+                                        int64_t* scores, void* stream) {
+  int st = check_switch(n, p, q, badness, scores);
+  if (count == 0) return N3_OK;
+  if (st) return st;
+  if (!candidates || count < 0) return set_error(N3_EINVAL, "bad candidates");
+  const uint16_t* table = nullptr;
+  DirTableInfo info{};
+  st = device_dirtable(n, &table, &info);
+  if (st) return st;
+  int* status = nullptr;
+  st = switch_status_word(&status);
+  if (st) return st;
+  hipStream_t s = as_stream(stream);
+  st = hip_status(hipMemsetAsync(status, 0, sizeof(int), s), "hipMemsetAsync(status)");
+  if (st) return st;
+  const int waves = 4;
+  const size_t lds = (size_t)table_lds_bytes(info) + (size_t)waves * info.hist_words * 4;
+  long long blocks = ((long long)count + waves - 1) / waves;
+  if (blocks > 1024) blocks = 1024;
+  const int ch = (2 * n + kWave - 1) / kWave;
+  void (*kernel)(const int32_t*, const int32_t*, int, const int32_t*, long long, long long, const long long*,
+                 long long*, int*, const uint16_t*, DirTableInfo) =
+      ch == 1 ? three_cycle_kernel<1> : (ch == 2 ? three_cycle_kernel<2> : (ch == 3 ? three_cycle_kernel<3> : three_cycle_kernel<4>));
+  hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(waves * kWave), lds, s, p, q, n, candidates,
+                     (long long)count, (long long)current_score, reinterpret_cast<const long long*>(badness),
+                     reinterpret_cast<long long*>(scores), status, table, info);
+  st = hip_status(hipGetLastError(), "three_cycle_kernel launch");
+  if (st) return st;
+  int host = 0;
+  st = hip_status(hipMemcpyAsync(&host, status, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(status)");
+  if (st) return st;
+  st = hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (st) return st;
+  if (host & 1) return set_error(N3_EINVAL, "a column lies outside [0, n)");
+  if (host & 2) return set_error(N3_EINVAL, "a candidate slot lies outside [0, 2n)");
+  return N3_OK;
 }

@@ -1,4 +1,4 @@
-"""2-switch move scores on the GPU.
+"""2-switch and 3-cycle move scores on the GPU.
 
 score_two_switches_gpu is the drop-in for gpu_delta.py:86-116 (int32 with
 the sentinel 1<<30); evaluate_two_switches / evaluate_transpositions keep the
@@ -57,6 +57,23 @@ def evaluate_transpositions(p, q, layer: int, selected_row: int, current_score: 
     _lib.check(_lib.lib().n3_evaluate_transpositions(tp.data_ptr(), tq.data_ptr(), tb.data_ptr(), n, int(layer),
                                                      int(selected_row), int(current_score), out.data_ptr(),
                                                      _lib.stream_handle(dev)))
+    return out.cpu().numpy()
+
+
+def evaluate_three_cycles(p, q, candidates, current_score: int, badness) -> np.ndarray:
+    """accelerated.py:198-276 computed on the GPU: int64 score per candidate
+    row (u, v, w, orientation)."""
+    p_host, q_host, bad_host = _state_arrays(p, q, badness, np.int64)
+    cand = np.ascontiguousarray(candidates, dtype=np.int32).reshape(-1, 4)
+    n = len(p_host)
+    if len(cand) == 0:
+        return np.empty(0, dtype=np.int64)
+    dev = _lib.device()
+    tp, tq, tb, tc = (torch.from_numpy(a).to(dev) for a in (p_host, q_host, bad_host, cand))
+    out = torch.empty(len(cand), dtype=torch.int64, device=dev)
+    _lib.check(_lib.lib().n3_evaluate_three_cycles(tp.data_ptr(), tq.data_ptr(), tb.data_ptr(), n, tc.data_ptr(),
+                                                   len(cand), int(current_score), out.data_ptr(),
+                                                   _lib.stream_handle(dev)))
     return out.cpu().numpy()
 
 

@@ -14,6 +14,7 @@
  *   n3o_score_slope              verifier_b.py:8-33     (normalized-direction counting form)
  *   n3o_evaluate_transpositions  accelerated.py:40-119
  *   n3o_evaluate_two_switches    accelerated.py:122-195
+ *   n3o_evaluate_three_cycles    accelerated.py:198-276
  *   n3o_walk (tempering replay)  tempering.py:48-110    (walk kernel step semantics)
  *   n3o_exchange                 tempering.py:169-191   (adjacent-temperature swaps)
  *   n3o_advance_learned          the walk above with learned first-slot proposals
@@ -382,6 +383,74 @@ void n3o_evaluate_two_switches(const int32_t* p, const int32_t* q, int n, int se
         if (cols[v] == cols[selected_other] || cols[u] == cols[partner_other]) continue;
         const int64_t old_affected = badness[u] + badness[v] - joint_old_count(rows, cols, m, u, v);
         out[v] = current_score - old_affected + new_affected_count(rows, cols, m, u, v);
+    }
+}
+
+/* accelerated.py:198-276: candidates (u, v, w, orientation); orientation 0
+ * moves u <- col v, v <- col w, w <- col u; orientation 1 moves u <- col w,
+ * w <- col v, v <- col u.  Sentinel 1<<60 is never produced (every candidate
+ * is scored, as in the reference). */
+void n3o_evaluate_three_cycles(const int32_t* p, const int32_t* q, int n, const int32_t* candidates, int64_t count,
+                               int64_t current_score, const int64_t* badness, int64_t* out)
+{
+    const int m = 2 * n;
+    int32_t rows[N3O_MAXM], cols[N3O_MAXM], changed[N3O_MAXM];
+    interleave(p, q, n, rows, cols);
+    for (int64_t ci = 0; ci < count; ++ci) {
+        const int u = candidates[4 * ci], v = candidates[4 * ci + 1], w = candidates[4 * ci + 2];
+        const int orientation = candidates[4 * ci + 3];
+        int64_t joint_uv = 0, joint_uw = 0, joint_vw = 0;
+        for (int o = 0; o < m; ++o) {
+            if (o != u && o != v &&
+                (rows[v] - rows[u]) * (cols[o] - cols[u]) == (cols[v] - cols[u]) * (rows[o] - rows[u]))
+                joint_uv += 1;
+            if (o != u && o != w &&
+                (rows[w] - rows[u]) * (cols[o] - cols[u]) == (cols[w] - cols[u]) * (rows[o] - rows[u]))
+                joint_uw += 1;
+            if (o != v && o != w &&
+                (rows[w] - rows[v]) * (cols[o] - cols[v]) == (cols[w] - cols[v]) * (rows[o] - rows[v]))
+                joint_vw += 1;
+        }
+        const int64_t triple_uvw =
+            (rows[v] - rows[u]) * (cols[w] - cols[u]) == (cols[v] - cols[u]) * (rows[w] - rows[u]) ? 1 : 0;
+        const int64_t old_affected =
+            badness[u] + badness[v] + badness[w] - joint_uv - joint_uw - joint_vw + triple_uvw;
+        for (int i = 0; i < m; ++i) changed[i] = cols[i];
+        if (orientation == 0) {
+            changed[u] = cols[v];
+            changed[v] = cols[w];
+            changed[w] = cols[u];
+        } else {
+            changed[u] = cols[w];
+            changed[w] = cols[v];
+            changed[v] = cols[u];
+        }
+        int64_t new_affected = 0;
+        for (int i = 0; i < m - 1; ++i) {
+            if (i == u) continue;
+            for (int j = i + 1; j < m; ++j) {
+                if (j == u) continue;
+                if ((rows[i] - rows[u]) * (changed[j] - changed[u]) == (changed[i] - changed[u]) * (rows[j] - rows[u]))
+                    new_affected += 1;
+            }
+        }
+        for (int i = 0; i < m - 1; ++i) {
+            if (i == u || i == v) continue;
+            for (int j = i + 1; j < m; ++j) {
+                if (j == u || j == v) continue;
+                if ((rows[i] - rows[v]) * (changed[j] - changed[v]) == (changed[i] - changed[v]) * (rows[j] - rows[v]))
+                    new_affected += 1;
+            }
+        }
+        for (int i = 0; i < m - 1; ++i) {
+            if (i == u || i == v || i == w) continue;
+            for (int j = i + 1; j < m; ++j) {
+                if (j == u || j == v || j == w) continue;
+                if ((rows[i] - rows[w]) * (changed[j] - changed[w]) == (changed[i] - changed[w]) * (rows[j] - rows[w]))
+                    new_affected += 1;
+            }
+        }
+        out[ci] = current_score - old_affected + new_affected;
     }
 }
 

@@ -54,6 +54,8 @@ def lib():
         L.n3o_evaluate_transpositions.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _i64p, _i64p]
         L.n3o_evaluate_two_switches.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _i64p, _i64p]
         L.n3o_delta_table.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_int64, _i32p]
+        L.n3o_evaluate_three_cycles.argtypes = [_i32p, _i32p, ctypes.c_int, _i32p, ctypes.c_int64, ctypes.c_int64,
+                                                _i64p, _i64p]
         L.n3o_metropolis_thresholds.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _u32p]
         L.n3o_exchange_thresholds.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, _i8p]
         L.n3o_advance.argtypes = [
@@ -136,6 +138,16 @@ def evaluate_two_switches(p, q, selected_index: int, current_score: int, badness
     p, q = _i32(p), _i32(q)
     out = np.empty(2 * len(p), dtype=np.int64)
     lib().n3o_evaluate_two_switches(p, q, len(p), selected_index, current_score,
+                                    np.ascontiguousarray(badness, dtype=np.int64), out)
+    return out
+
+
+def evaluate_three_cycles(p, q, candidates, current_score: int, badness) -> np.ndarray:
+    """accelerated.py:198-276 (int64 per candidate row (u, v, w, orientation))."""
+    p, q = _i32(p), _i32(q)
+    cand = np.ascontiguousarray(candidates, dtype=np.int32).reshape(-1, 4)
+    out = np.empty(len(cand), dtype=np.int64)
+    lib().n3o_evaluate_three_cycles(p, q, len(p), cand, len(cand), current_score,
                                     np.ascontiguousarray(badness, dtype=np.int64), out)
     return out
 

@@ -112,3 +112,37 @@ def test_delta_shape_errors(pkg, dev):
         pkg.score_two_switches_gpu(np.arange(5), np.arange(5), 0, 0, np.zeros(9))
     with pytest.raises(ValueError):
         pkg.evaluate_two_switches(np.arange(5), np.roll(np.arange(5), 1), 10, 0, np.zeros(10))
+
+
+@pytest.mark.parametrize("n", [4, 9, 20, 32, 33, 75, 100])
+def test_three_cycles(pkg, oracle, dev, n):
+    hp, hq = oracle.generate_states(500 + n, n, 3)
+    rng = np.random.default_rng(n)
+    for s in range(3):
+        p, q = hp[s], hq[s]
+        score, bad = oracle.score_and_badness(p, q)
+        u = int(rng.integers(0, 2 * n))
+        cand = pkg.legal_three_cycle_candidates(p, q, u)
+        if len(cand) > 300:
+            cand = cand[rng.choice(len(cand), 300, replace=False)]
+        assert np.array_equal(pkg.evaluate_three_cycles(p, q, cand, score, bad),
+                              oracle.evaluate_three_cycles(p, q, cand, score, bad))
+
+
+def test_three_cycles_reference_states(pkg, oracle, dev):
+    """The states of the reference's test_three_cycle_scores_match_full_recompute."""
+    from random import Random
+
+    rng = Random(991)
+    for n in range(4, 10):
+        state = pkg.random_state(n, rng)
+        p = np.asarray(state.P, dtype=np.int32)
+        q = np.asarray(state.Q, dtype=np.int32)
+        score, bad = oracle.score_and_badness(p, q)
+        selected = rng.randrange(2 * n)
+        cand = pkg.legal_three_cycle_candidates(p, q, selected)
+        if len(cand) == 0:
+            continue
+        sample = cand[: min(30, len(cand))]
+        assert np.array_equal(pkg.evaluate_three_cycles(p, q, sample, score, bad),
+                              oracle.evaluate_three_cycles(p, q, sample, score, bad))

@@ -94,3 +94,27 @@ def test_points_to_permutations(pkg):
     # matching colourings keep the point set (tests/test_state.py:34-38)
     s = pkg.PairPermutations((0, 1, 2, 3, 4, 5), (1, 0, 3, 2, 5, 4))
     assert pkg.points_to_permutations(s.points(), 6).points() == s.points()
+
+
+def test_three_cycle_candidates_match_reference_loop(pkg):
+    import importlib
+
+    helper = importlib.import_module("test_oracle")._three_cycle_candidates
+    rng = np.random.default_rng(8)
+    for n in (3, 4, 7, 12, 30):
+        for _ in range(3):
+            p = rng.permutation(n).astype(np.int32)
+            q = rng.permutation(n).astype(np.int32)
+            for u in (0, 2 * n - 1, int(rng.integers(0, 2 * n))):
+                assert np.array_equal(pkg.legal_three_cycle_candidates(p, q, u), helper(p, q, u))
+
+
+def test_apply_three_cycle(pkg):
+    state = pkg.random_state(9, __import__("random").Random(3))
+    p, q = np.asarray(state.P), np.asarray(state.Q)
+    cand = pkg.legal_three_cycle_candidates(p, q, 4)
+    assert len(cand) > 0
+    moved = pkg.apply_three_cycle(p, q, cand[0])
+    moved.validate()
+    before, after = set(state.points()), set(moved.points())
+    assert len(before - after) == 3 and len(after - before) == 3

@@ -264,3 +264,54 @@ def test_exchange_thresholds_law(oracle):
     assert list(sign) == [1, 1, 0, 0]
     c = 1 / np.float64(np.float32(0.1)) - 1 / np.float64(np.float32(0.2))
     assert abs(thr[0, 3] / 2**32 - np.exp(-c * 3)) < 2**-31
+
+
+def _three_cycle_candidates(p, q, u):
+    """search.py:38-65 as a plain loop (test helper)."""
+    n = len(p)
+    cols = [0] * (2 * n)
+    for r in range(n):
+        cols[2 * r], cols[2 * r + 1] = int(p[r]), int(q[r])
+    ru = u // 2
+    out = []
+    for v in range(2 * n - 1):
+        if v // 2 == ru:
+            continue
+        for w in range(v + 1, 2 * n):
+            if w // 2 == ru or w // 2 == v // 2:
+                continue
+            if len({cols[u], cols[v], cols[w]}) != 3:
+                continue
+            ou, ov, ow = cols[u ^ 1], cols[v ^ 1], cols[w ^ 1]
+            if cols[v] != ou and cols[w] != ov and cols[u] != ow:
+                out.append((u, v, w, 0))
+            if cols[w] != ou and cols[v] != ow and cols[u] != ov:
+                out.append((u, v, w, 1))
+    return np.asarray(out, np.int32).reshape(-1, 4)
+
+
+def test_three_cycles_match_full_recompute(oracle):
+    """tests/test_delta.py:63-83 of the reference (Random(991)), with every
+    candidate recounted by the line formulation."""
+    rng = Random(991)
+    comparisons = 0
+    for n in range(4, 10):
+        P, Q = _random_state(n, rng)
+        p = np.asarray(P, np.int32)
+        q = np.asarray(Q, np.int32)
+        score, badness = oracle.score_and_badness(p, q)
+        selected = rng.randrange(2 * n)
+        cand = _three_cycle_candidates(p, q, selected)
+        if len(cand) == 0:
+            continue
+        sample = cand[: min(30, len(cand))]
+        scores = oracle.evaluate_three_cycles(p, q, sample, score, badness)
+        pts = slots(p, q)
+        for (u, v, w, o), s in zip(sample, scores):
+            moved = list(pts)
+            cu, cv, cw = pts[u][1], pts[v][1], pts[w][1]
+            nu, nv, nw = (cv, cw, cu) if o == 0 else (cw, cu, cv)
+            moved[u], moved[v], moved[w] = (pts[u][0], nu), (pts[v][0], nv), (pts[w][0], nw)
+            assert s == line_count(moved)[0]
+            comparisons += 1
+    assert comparisons >= 50

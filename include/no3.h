@@ -127,9 +127,12 @@ typedef struct n3_chains {
     int64_t* proposals;      /* [2]: valid proposals (total), accepted moves (total) */
     int block_waves;         /* replicas (wavefronts) per workgroup; 0 = automatic */
     int32_t* log;            /* optional proposal log (NULL = off): per valid proposal
-                                (badness u, badness v, energy, slot, delta, accepted) */
+                                (badness u, badness v, energy, slot, delta, accepted);
+                                each replica reserves `steps` consecutive entries per
+                                sweep (entry = reservation + step); invalid proposals
+                                and reservations past the capacity write nothing */
     int64_t log_capacity;    /* entries */
-    uint64_t* log_count;     /* entries written (may exceed the capacity) */
+    uint64_t* log_count;     /* entries reserved (may exceed the capacity) */
 } n3_chains;
 
 /* `steps` proposals per replica (one sweep), TemperingChains.advance

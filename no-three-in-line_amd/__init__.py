@@ -24,7 +24,7 @@ from .gpu_delta import (
     score_two_switches_gpu,
 )
 from .moves import apply_three_cycle, legal_three_cycle_candidates
-from .state import PairPermutations, points_to_permutations, random_state
+from .state import PairPermutations, points_to_permutations, random_state, random_two_switches
 from .tempering import TemperingChains
 from .batched import BatchGroup, run_batched, seeded_chains
 from . import autotune, learned
@@ -34,6 +34,6 @@ __all__ = [
     "check_status", "generate_states", "score_population_badness", "score_population_device", "score_population_gpu",
     "delta_table", "delta_table_device", "evaluate_three_cycles", "evaluate_transpositions",
     "apply_three_cycle", "legal_three_cycle_candidates", "evaluate_two_switches", "score_two_switches_gpu",
-    "PairPermutations", "points_to_permutations", "random_state", "TemperingChains", "_lib",
+    "PairPermutations", "points_to_permutations", "random_state", "random_two_switches", "TemperingChains", "_lib",
     "BatchGroup", "run_batched", "seeded_chains", "autotune", "learned",
 ]

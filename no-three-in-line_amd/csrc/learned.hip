@@ -14,9 +14,10 @@
 // with r the move's 32-bit Philox word and thr64(d) = floor(2^32 exp(-d/T))
 // (saturating at 2^63), evaluated in 128-bit integers -- so a CPU replay with
 // the same weight table reproduces every trajectory bit for bit.  Evaluating
-// the reverse probability needs the badness of every slot in x', so every
-// valid proposal runs the four direction histograms that the uniform walk
-// only needs after an acceptance.
+// the reverse probability needs the badness of every slot in x' (two more
+// direction histograms, from u and v in x); a proposal is rejected before
+// them when an upper bound of the Hastings ratio, from quantities already
+// known, rejects it (the exact test would reject as well).
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -38,22 +39,30 @@ struct LearnedArgs {
   int k64;
 };
 
+// Weight of a slot with collinear-pair count `count` from a weight row held
+// one entry per lane: per-lane counts use a lane permute, uniform counts a
+// scalar read.
 __device__ inline uint32_t weight_of(uint32_t row, int count) { return (uint32_t)__shfl((int)row, count < 63 ? count : 63, kWave); }
+__device__ inline uint32_t weight_at(uint32_t row, int count) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)row, count < 63 ? count : 63);
+}
 
 template <int CH>
 __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, const uint16_t* table, uint32_t* hist,
-                                     const DirTableInfo& info, int slot) {
+                                     const DirTableInfo& info, int slot_in) {
   const int lane = lane_id();
+  const int slot = __builtin_amdgcn_readfirstlane(slot_in);  // uniform: scalar registers
   const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
   const uint32_t sweep = a.counters[0];
   const long long base = (long long)slot * m;
 
-  int key[CH], bad[CH];
+  int key[CH], bad[CH], ref[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int k = lane + kWave * c;
     key[c] = k < m ? (k >> 1) * width + a.columns[base + k] : 0;
     bad[c] = k < m ? a.badness[base + k] : 0;
+    ref[c] = k < m ? a.reference[k] : -1;
   }
   {
     const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
@@ -69,6 +78,9 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
   unsigned long long valid_moves = 0;
   const uint32_t* wt = L.weights + (long long)L.temp_index[slot] * (L.max_energy + 1) * 64;
   const unsigned long long* thr = L.thr64 + (long long)slot * 2 * L.k64;
+  // acceptance words thr[j], j = d + k64 < 128, in registers
+  const unsigned long long thr_lo = lane < 2 * L.k64 ? thr[lane] : 0ull;
+  const unsigned long long thr_hi = lane + kWave < 2 * L.k64 ? thr[lane + kWave] : 0ull;
   auto weight_row = [&](int energy) -> uint32_t {
     const int e = energy < 0 ? 0 : (energy > L.max_energy ? L.max_energy : energy);
     return wt[(long long)e * 64 + lane];
@@ -103,12 +115,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
       uint32_t carry = 0;
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
-        uint32_t x = (uint32_t)w[c];
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          const uint32_t y = (uint32_t)__shfl_up((int)x, off, kWave);
-          if (lane >= off) x += y;
-        }
+        const uint32_t x = wave_scan_dpp((uint32_t)w[c]);
         const int k = lane + kWave * c;
         below += __popcll(__ballot(k < m && carry + x <= target));
         carry += (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
@@ -125,11 +132,11 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
     if (!valid) continue;
     valid_moves += 1;
-    const int ur = 2 * ru, vr = 2 * rv;
-    const int dd = (cu == a.reference[ur] || cu == a.reference[ur + 1]) +
-                   (cv == a.reference[vr] || cv == a.reference[vr + 1]) -
-                   (cv == a.reference[ur] || cv == a.reference[ur + 1]) -
-                   (cu == a.reference[vr] || cu == a.reference[vr + 1]);
+    int ref_u0, ref_u1, ref_v0, ref_v1;
+    slot_pair<CH>(ref, ru, ref_u0, ref_u1);
+    slot_pair<CH>(ref, rv, ref_v0, ref_v1);
+    const int dd = (cu == ref_u0 || cu == ref_u1) + (cv == ref_v0 || cv == ref_v1) - (cv == ref_u0 || cv == ref_u1) -
+                   (cu == ref_v0 || cu == ref_v1);
 
     const int id_uv = table[key_v - key_u + offset];
     int idu[CH];
@@ -178,6 +185,28 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     const int bad_u2 = (int)((packed & 0xFFFFu) >> 1), bad_v2 = (int)((packed >> 16) >> 1);
     const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
     const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
+    const int d = delta < -L.k64 ? -L.k64 : (delta > L.k64 - 1 ? L.k64 - 1 : delta);
+    const unsigned long long t64 = register_table64(thr_lo, thr_hi, thr, d + L.k64);
+    const uint32_t row2 = weight_row(score + delta);
+    const uint64_t wu = (uint32_t)slot_value<CH>(w, u), wv = (uint32_t)slot_value<CH>(w, v);
+    const uint64_t big_n = 2ull * s1 + 4ull * (uint64_t)m * (wu + wv);
+    if (t64 < (1ull << 32)) {
+      // Early rejection: with X = w'(u) + w'(v) (known now) and
+      // S1' >= B = X + (m - 2) min(row), N'/S1' = 2 + 4 m X / S1' <= A / B,
+      // A = 2 B + 4 m X, so r N B >= thr S1 A implies r N S1' >= thr N' S1,
+      // i.e. the full test below would reject.  The products (< 2^93) are
+      // formed in fp32: at most 13 roundings in all (conversions, products,
+      // the sum in A, the margin product) bound the error of the ratio of the
+      // two sides by (1 + 2^-24)^13 < 1 + 2^-19, so rejecting only when the
+      // left side exceeds the right by the factor 1 + 2^-13 never rejects a
+      // move the exact test would accept; moves inside the margin go on.
+      const uint32_t x2 = weight_at(row2, bad_u2) + weight_at(row2, bad_v2);
+      const uint32_t lower = x2 + (uint32_t)(m - 2) * wave_min_dpp(row2);
+      const float upper = 2.0f * (float)lower + 4.0f * (float)m * (float)x2;
+      const float lhs = (float)r[3] * (float)big_n * (float)lower;
+      const float rhs = (float)t64 * (float)s1 * upper;
+      if (lhs > rhs * (1.0f + 0x1p-13f)) continue;
+    }
 
     // badness of every slot in the moved state (needed for the reverse move)
 #pragma unroll
@@ -187,7 +216,6 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
     direction_counts<CH>(table, hist, trash, offset, key_v, key, include, cnt_v, id);
     int nb[CH], w2[CH];
-    const uint32_t row2 = weight_row(score + delta);
     uint32_t part2 = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
@@ -207,13 +235,10 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
       part2 += (uint32_t)w2[c];
     }
     const uint32_t s1_new = wave_sum_dpp(part2);
-    const uint64_t wu = (uint32_t)slot_value<CH>(w, u), wv = (uint32_t)slot_value<CH>(w, v);
     const uint64_t wu2 = (uint32_t)slot_value<CH>(w2, u), wv2 = (uint32_t)slot_value<CH>(w2, v);
-    const uint64_t big_n = 2ull * s1 + 4ull * (uint64_t)m * (wu + wv);
     const uint64_t big_n2 = 2ull * s1_new + 4ull * (uint64_t)m * (wu2 + wv2);
-    const int d = delta < -L.k64 ? -L.k64 : (delta > L.k64 - 1 ? L.k64 - 1 : delta);
     const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * s1_new;
-    const unsigned __int128 rhs = (unsigned __int128)thr[d + L.k64] * big_n2 * s1;
+    const unsigned __int128 rhs = (unsigned __int128)t64 * big_n2 * s1;
     if (!(lhs < rhs)) continue;
 
 #pragma unroll

@@ -38,18 +38,23 @@ namespace no3 {
 // wavefront's LDS direction table and histogram.
 template <int CH>
 __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t* hist, const DirTableInfo& info,
-                             int slot) {
+                             int slot_in) {
   const int lane = lane_id();
+  // the slot is the same on every lane: keep it (and the Philox counters,
+  // proposals and table indexes derived from it) in scalar registers
+  const int slot = __builtin_amdgcn_readfirstlane(slot_in);
   const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
   const uint32_t sweep = a.counters[0];
   const long long base = (long long)slot * m;
 
-  int key[CH], bad[CH];
+  // per lane: keys, badness and reference columns of slots lane + 64 c
+  int key[CH], bad[CH], ref[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int k = lane + kWave * c;
     key[c] = k < m ? (k >> 1) * width + a.columns[base + k] : 0;
     bad[c] = k < m ? a.badness[base + k] : 0;
+    ref[c] = k < m ? a.reference[k] : -1;
   }
   // lanes past the last slot hold slot 0's key (always a valid point)
   {
@@ -65,6 +70,20 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
   int accepted = 0;
   unsigned long long valid_moves = 0;
   const uint32_t* thr = a.thr + (long long)slot * a.k_move;
+  // Metropolis words thr[d] for d < 128 in two registers per lane
+  const uint32_t thr_lo = lane < a.k_move ? thr[lane] : 0u;
+  const uint32_t thr_hi = lane + kWave < a.k_move ? thr[lane + kWave] : 0u;
+  // proposal log: the wavefront reserves `steps` entries for this sweep with
+  // one atomic (entry log_base + step); a reservation that does not fit in
+  // the capacity logs nothing, and invalid proposals leave their entry as is
+  int32_t* log_base = nullptr;
+  if (a.log) {
+    unsigned long long first = 0;
+    if (lane == 0) first = atomicAdd(a.log_count, (unsigned long long)a.steps);
+    first = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(first >> 32), 0) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane((int)first, 0);
+    if (first + (unsigned long long)a.steps <= (unsigned long long)a.log_capacity) log_base = a.log + 6 * first;
+  }
 
   for (int step = 0; step < a.steps; ++step) {
     uint32_t r[4];
@@ -79,12 +98,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         // inclusive prefix of badness over the slots of this chunk
-        int x = bad[c];
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          const int y = __shfl_up(x, off, kWave);
-          if (lane >= off) x += y;
-        }
+        const int x = (int)wave_scan_dpp((uint32_t)bad[c]);
         const int k = lane + kWave * c;
         below += __popcll(__ballot(k < m && carry + x <= target));
         carry += __builtin_amdgcn_readlane(x, 63);
@@ -100,11 +114,11 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
     if (!valid) continue;
     valid_moves += 1;
-    const int ur = 2 * ru, vr = 2 * rv;
-    const int dd = (cu == a.reference[ur] || cu == a.reference[ur + 1]) +
-                   (cv == a.reference[vr] || cv == a.reference[vr + 1]) -
-                   (cv == a.reference[ur] || cv == a.reference[ur + 1]) -
-                   (cu == a.reference[vr] || cu == a.reference[vr + 1]);
+    int ref_u0, ref_u1, ref_v0, ref_v1;
+    slot_pair<CH>(ref, ru, ref_u0, ref_u1);
+    slot_pair<CH>(ref, rv, ref_v0, ref_v1);
+    const int dd = (cu == ref_u0 || cu == ref_u1) + (cv == ref_v0 || cv == ref_v1) - (cv == ref_u0 || cv == ref_u1) -
+                   (cu == ref_v0 || cu == ref_v1);
 
     // joint(u, v) in the current state
     const int id_uv = table[key_v - key_u + offset];
@@ -159,20 +173,17 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
 
     // Metropolis on the integer delta (tempering.py:91-95)
     bool accept = delta <= 0;
-    if (!accept) accept = r[2] < thr[delta < a.k_move ? delta : a.k_move - 1];
-    if (a.log && lane == 0) {
+    if (!accept) accept = r[2] < register_table(thr_lo, thr_hi, thr, delta < a.k_move ? delta : a.k_move - 1);
+    if (log_base && lane == 0) {
       // (badness u, badness v, energy, slot, delta, accepted): training data
       // for the proposal network
-      const unsigned long long i = atomicAdd(a.log_count, 1ull);
-      if (i < (unsigned long long)a.log_capacity) {
-        int32_t* e = a.log + 6 * i;
-        e[0] = bad_u;
-        e[1] = bad_v;
-        e[2] = score;
-        e[3] = slot;
-        e[4] = delta;
-        e[5] = accept ? 1 : 0;
-      }
+      int32_t* e = log_base + 6 * step;
+      e[0] = bad_u;
+      e[1] = bad_v;
+      e[2] = score;
+      e[3] = slot;
+      e[4] = delta;
+      e[5] = accept ? 1 : 0;
     }
     if (!accept) continue;
 

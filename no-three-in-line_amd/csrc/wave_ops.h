@@ -20,6 +20,34 @@ __device__ inline uint32_t wave_sum_dpp(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
+// Inclusive prefix sum over the 64 lanes in lane order, on the VALU:
+// row_shr 1, 2, 4, 8 inside each row of 16 (lanes whose source lies before
+// the row start add 0), then row 0's total into row 1 and row 2's into row 3
+// (row_bcast:15), then the total of rows 0-1 into rows 2 and 3
+// (row_bcast:31).
+__device__ inline uint32_t wave_scan_dpp(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return x;
+}
+
+// Minimum over the 64 lanes, by the same DPP pattern as wave_sum_dpp (lanes
+// a step does not write keep the identity 0xFFFFFFFF).
+__device__ inline uint32_t wave_min_dpp(uint32_t x) {
+  constexpr int kIdentity = -1;
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0xB1, 0xF, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x4E, 0xF, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x141, 0xF, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x140, 0xF, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x142, 0xA, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x143, 0xC, 0xF, false));
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Value of a per-lane register array at a uniform slot (slot = lane + 64 c).
 template <int CH>
 __device__ inline int slot_value(const int (&arr)[CH], int slot) {
@@ -31,6 +59,44 @@ __device__ inline int slot_value(const int (&arr)[CH], int slot) {
     r = (chunk == c) ? t : r;
   }
   return r;
+}
+
+// Values of a per-lane register array at the uniform slots 2r and 2r + 1
+// (one chunk, neighbouring lanes).
+template <int CH>
+__device__ inline void slot_pair(const int (&arr)[CH], int row, int& first, int& second) {
+  const int slot = 2 * row, lane = slot & (kWave - 1), chunk = slot >> 6;
+  int a = __builtin_amdgcn_readlane(arr[0], lane), b = __builtin_amdgcn_readlane(arr[0], lane + 1);
+#pragma unroll
+  for (int c = 1; c < CH; ++c) {
+    const int ta = __builtin_amdgcn_readlane(arr[c], lane), tb = __builtin_amdgcn_readlane(arr[c], lane + 1);
+    a = (chunk == c) ? ta : a;
+    b = (chunk == c) ? tb : b;
+  }
+  first = a;
+  second = b;
+}
+
+// Entry `index` (uniform, >= 0) of a small per-replica table `global` whose
+// first 128 entries the wavefront holds in registers, entry lane + 64 j in
+// register j (lo: j = 0, hi: j = 1); later entries are read from memory.
+__device__ inline uint32_t register_table(uint32_t lo, uint32_t hi, const uint32_t* global, int index) {
+  if (index < kWave) return (uint32_t)__builtin_amdgcn_readlane((int)lo, index);
+  if (index < 2 * kWave) return (uint32_t)__builtin_amdgcn_readlane((int)hi, index - kWave);
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)global[index]);
+}
+
+__device__ inline unsigned long long register_table64(unsigned long long lo, unsigned long long hi,
+                                                      const unsigned long long* global, int index) {
+  auto read = [](unsigned long long x, int l) {
+    return ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), l) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)x, l);
+  };
+  if (index < kWave) return read(lo, index);
+  if (index < 2 * kWave) return read(hi, index - kWave);
+  const unsigned long long x = global[index];
+  return ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
+         (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
 }
 
 // Histogram of the directions from `anchor` (a key) to the points keys[c] of

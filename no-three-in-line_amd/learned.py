@@ -57,6 +57,18 @@ def point_features(counts, log_t, energy) -> np.ndarray:
     return f
 
 
+def point_features_device(counts: torch.Tensor, log_t: torch.Tensor, energy: torch.Tensor) -> torch.Tensor:
+    """point_features on device tensors (float32 [N] each) -> float32 [N, 8]."""
+    b = counts.to(torch.float32)
+    e = energy.to(torch.float32)
+    lt = log_t.to(torch.float32)
+    third = torch.tensor(1.0 / 3.0, dtype=torch.float32, device=b.device)
+    return torch.stack((b * 0.125, torch.sqrt(b) * third, torch.log1p(b) * third, (b > 0).to(torch.float32), lt,
+                        e * torch.tensor(1.0 / 150.0, dtype=torch.float32, device=b.device),
+                        torch.sqrt(e) * torch.tensor(1.0 / 12.0, dtype=torch.float32, device=b.device),
+                        b * 0.125 * lt), dim=1)
+
+
 class ProposalNetwork:
     """Parameters on the device: bf16 copy used by the kernels, fp32 master
     copy and Adam moments."""
@@ -87,35 +99,52 @@ class ProposalNetwork:
                                              self._stream()))
         return out
 
-    def gradient(self, xu: torch.Tensor, xv: torch.Tensor, y: torch.Tensor) -> tuple[torch.Tensor, float]:
+    def gradient(self, xu: torch.Tensor, xv: torch.Tensor, y: torch.Tensor,
+                 loss: torch.Tensor | None = None) -> tuple[torch.Tensor, float | None]:
+        """Gradient of the mean pair loss.  With `loss` given, the summed loss
+        is added to it on the device (no synchronisation) and None returned
+        in place of the mean loss."""
         grad = torch.zeros(PARAMS, dtype=torch.float32, device=self.device)
-        loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+        acc = torch.zeros(1, dtype=torch.float32, device=self.device) if loss is None else loss
         _lib.check(_lib.lib().n3_mlp_grad(self.params.data_ptr(), xu.data_ptr(), xv.data_ptr(), y.data_ptr(),
-                                          int(y.shape[0]), grad.data_ptr(), loss.data_ptr(), self._stream()))
-        return grad, float(loss.item()) / int(y.shape[0])
+                                          int(y.shape[0]), grad.data_ptr(), acc.data_ptr(), self._stream()))
+        if loss is not None:
+            return grad, None
+        return grad, float(acc.item()) / int(y.shape[0])
 
-    def adam(self, grad: torch.Tensor, lr: float = 3e-3) -> None:
+    def adam(self, grad: torch.Tensor, lr: float = 1e-2) -> None:
         self.step += 1
         _lib.check(_lib.lib().n3_mlp_adam(self.master.data_ptr(), self.params.data_ptr(), grad.data_ptr(),
                                           self.m.data_ptr(), self.v.data_ptr(), self.step, lr, 0.9, 0.999, 1e-8,
                                           self._stream()))
 
-    def train(self, xu: torch.Tensor, xv: torch.Tensor, y: torch.Tensor, *, epochs: int = 2, batch: int = 32768,
-              lr: float = 3e-3, seed: int = 0) -> list[float]:
-        """Mini-batch Adam on (xu, xv, y); returns the mean loss per epoch."""
-        gen = torch.Generator(device="cpu").manual_seed(seed)
-        losses = []
+    def train(self, xu: torch.Tensor, xv: torch.Tensor, y: torch.Tensor, *, steps: int | None = None,
+              epochs: int = 2, batch: int = 32768, lr: float = 1e-2, seed: int = 0, reports: int = 4) -> list[float]:
+        """Mini-batch Adam on (xu, xv, y).  With `steps`, that many batches
+        drawn with replacement on the device; otherwise `epochs` passes over a
+        permutation.  Returns the mean loss over each of `reports` equal
+        parts of the run (one synchronisation at the end)."""
         total = int(y.shape[0])
-        for _ in range(epochs):
-            order = torch.randperm(total, generator=gen).to(self.device)
-            epoch_loss = 0.0
-            for start in range(0, total, batch):
-                idx = order[start:start + batch]
-                grad, loss = self.gradient(xu[idx].contiguous(), xv[idx].contiguous(), y[idx].contiguous())
-                self.adam(grad, lr)
-                epoch_loss += loss * len(idx)
-            losses.append(epoch_loss / total)
-        return losses
+        if total == 0:
+            raise ValueError("no training tuples")
+        gen = torch.Generator(device=self.device).manual_seed(seed)
+        batches = []
+        if steps is not None:
+            for _ in range(int(steps)):
+                batches.append(torch.randint(0, total, (min(batch, total),), generator=gen, device=self.device))
+        else:
+            for _ in range(int(epochs)):
+                order = torch.randperm(total, generator=gen, device=self.device)
+                batches.extend(order[start:start + batch] for start in range(0, total, batch))
+        parts = max(1, min(int(reports), len(batches)))
+        sums = torch.zeros(parts, dtype=torch.float32, device=self.device)
+        counts = [0] * parts
+        for i, idx in enumerate(batches):
+            part = i * parts // len(batches)
+            grad, _ = self.gradient(xu[idx], xv[idx], y[idx], loss=sums[part:part + 1])
+            counts[part] += len(idx)
+            self.adam(grad, lr)
+        return [float(v) / c for v, c in zip(sums.cpu().tolist(), counts)]
 
     def weight_table(self, log_temps: np.ndarray, max_energy: int = MAX_ENERGY) -> torch.Tensor:
         lt = torch.from_numpy(np.ascontiguousarray(log_temps, dtype=np.float32)).to(self.device)
@@ -126,31 +155,55 @@ class ProposalNetwork:
 
 
 class ProposalLog:
-    """Device ring of logged proposals (bad_u, bad_v, energy, slot, delta,
-    accepted), filled by uniform-proposal walks while attached to chains."""
+    """Device log of proposals (bad_u, bad_v, energy, slot, delta, accepted),
+    filled by uniform-proposal walks while attached to chains.  Every replica
+    reserves one entry per proposal of a sweep; entries of invalid proposals
+    keep the fill value -1, and sweeps whose reservation does not fit are not
+    logged."""
 
     def __init__(self, capacity: int, device: torch.device):
         self.capacity = int(capacity)
-        self.entries = torch.zeros((self.capacity, 6), dtype=torch.int32, device=device)
+        self.entries = torch.full((self.capacity, 6), -1, dtype=torch.int32, device=device)
         self.count = torch.zeros(1, dtype=torch.int64, device=device)
 
     def attach(self, chains: TemperingChains) -> None:
         chains.log = self
 
+    def reset(self) -> None:
+        self.entries.fill_(-1)
+        self.count.zero_()
+
     def rows(self) -> torch.Tensor:
         filled = min(int(self.count.item()), self.capacity)
-        return self.entries[:filled]
+        head = self.entries[:filled]
+        return head[head[:, 3] >= 0]
 
 
-def training_tensors(entries: torch.Tensor, temperatures: np.ndarray, device) -> tuple:
-    """Feature rows of the two moved points and the accepted labels."""
-    e = entries.cpu().numpy()
-    log_t = np.log(temperatures.astype(np.float32))[e[:, 3]]
-    xu = to_bf16(point_features(e[:, 0], log_t, e[:, 2])).view(np.int16)
-    xv = to_bf16(point_features(e[:, 1], log_t, e[:, 2])).view(np.int16)
-    y = e[:, 5].astype(np.float32)
-    return (torch.from_numpy(np.ascontiguousarray(xu)).to(device),
-            torch.from_numpy(np.ascontiguousarray(xv)).to(device), torch.from_numpy(y).to(device))
+def training_tensors(entries: torch.Tensor, temperatures: np.ndarray, device, *, label: str = "metropolis",
+                     temperature_floor: float = 0.25) -> tuple:
+    """Feature rows (bf16 bits) of the two moved points and their labels,
+    computed on the device.
+
+    label "accepted" is the logged outcome (0/1).  label "metropolis" is the
+    acceptance probability min(1, exp(-delta / max(T, temperature_floor)))
+    of the logged exact delta: the expectation of the outcome (at T >= the
+    floor) without its sampling noise, and, below the floor, the acceptance
+    at the floor, since the outcomes there are almost all rejections and
+    carry little signal about which points are worth moving."""
+    e = entries.to(device)
+    temps = torch.from_numpy(np.ascontiguousarray(temperatures, dtype=np.float32)).to(device)
+    slot = e[:, 3].long()
+    lt = torch.log(temps)[slot]
+    xu = point_features_device(e[:, 0], lt, e[:, 2]).to(torch.bfloat16).view(torch.int16).contiguous()
+    xv = point_features_device(e[:, 1], lt, e[:, 2]).to(torch.bfloat16).view(torch.int16).contiguous()
+    if label == "accepted":
+        y = e[:, 5].to(torch.float32)
+    elif label == "metropolis":
+        t_eff = torch.clamp(temps[slot], min=float(temperature_floor))
+        y = torch.exp(-torch.clamp(e[:, 4].to(torch.float32), min=0.0) / t_eff)
+    else:
+        raise ValueError(f"unknown label {label!r}")
+    return xu, xv, y.contiguous()
 
 
 @dataclass
@@ -210,24 +263,42 @@ def threshold_times(trace: list[tuple[float, int]], thresholds) -> dict:
 
 
 def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, state_seed: int,
-           train_sweeps: int = 200, log_capacity: int = 1 << 21, chunk: int = 50,
-           thresholds=(80, 70, 60, 55, 50, 45, 40, 12), tmin: float = 0.05, tmax: float = 2.0,
-           device: torch.device | None = None) -> dict:
+           train_sweeps: int = 200, log_sweeps: int = 2, train_steps: int = 400, retrain_seconds: float = 2.0,
+           retrain_steps: int = 100, log_capacity: int = 1 << 21,
+           chunk: int = 50, thresholds=(80, 70, 60, 55, 50, 45, 40, 12), tmin: float = 0.05, tmax: float = 2.0,
+           start: np.ndarray | None = None, device: torch.device | None = None) -> dict:
     """Uniform vs learned proposals on the same seeds, replicas and GPU.
 
-    Both start from the same seeded states.  The uniform run proceeds in
-    graph-captured chunks; the learned run first runs `train_sweeps` plain
-    sweeps that log proposals, trains the network on them (timed), then
-    continues with learned proposals.  The trace records (elapsed seconds,
-    best energy) after every chunk; training time counts against the learned
-    run."""
+    Both start from the same states: seeded generator states, or the rows of
+    `start` (int32[replicas][2n]) when given.  The uniform run proceeds in
+    graph-captured chunks.  The learned run first runs `train_sweeps` plain
+    sweeps, the last `log_sweeps` of them logged, trains the network on that
+    log (`train_steps` Adam steps) and continues with learned proposals; every
+    `retrain_seconds` it logs `log_sweeps` more plain sweeps and takes
+    `retrain_steps` more steps on them (online training).  The trace records (elapsed seconds, best energy) after every
+    chunk; logging and training time count against the learned run."""
     from .batched import seeded_chains
 
     dev = _lib.device(None if device is None else device.index)
     steps = 2 * n
+    if not 0 < log_sweeps <= train_sweeps:
+        raise ValueError("need 0 < log_sweeps <= train_sweeps")
+    if log_sweeps * steps * replicas > log_capacity:
+        raise ValueError("log_capacity is too small for log_sweeps")
     result = {}
+
+    def chains():
+        if start is None:
+            return seeded_chains(n, replicas, ladder, seed=seed, state_seed=state_seed, tmin=tmin, tmax=tmax,
+                                 device=dev)
+        cols = np.ascontiguousarray(start, dtype=np.int32)
+        if cols.shape != (replicas, 2 * n):
+            raise ValueError("start must have shape (replicas, 2n)")
+        temps = np.tile(np.geomspace(tmin, tmax, ladder), replicas // ladder).astype(np.float32)
+        return TemperingChains(cols, temps, seed, device=dev)
+
     # uniform
-    uni = seeded_chains(n, replicas, ladder, seed=seed, state_seed=state_seed, tmin=tmin, tmax=tmax, device=dev)
+    uni = chains()
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     trace_u = []
@@ -239,35 +310,58 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
     result["uniform"] = {"trace": trace_u, "sweeps": chunk * len(trace_u),
                          "threshold_seconds": threshold_times(trace_u, thresholds)}
     # learned
-    lrn = seeded_chains(n, replicas, ladder, seed=seed, state_seed=state_seed, tmin=tmin, tmax=tmax, device=dev)
+    lrn = chains()
     net = ProposalNetwork(seed=seed, device=dev)
     log = ProposalLog(log_capacity, dev)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     trace_l = []
-    lrn.log = log
+    sweeps = 0
+
+    def plain(count, logged):
+        nonlocal sweeps
+        lrn.log = log if logged else None
+        lrn.run(count, steps, ladder)
+        lrn.log = None
+        sweeps += count
+
     done = 0
-    while done < train_sweeps:
-        lrn.run(min(chunk, train_sweeps - done), steps, ladder)
-        done += min(chunk, train_sweeps - done)
+    while done < train_sweeps - log_sweeps:
+        step = min(chunk, train_sweeps - log_sweeps - done)
+        plain(step, False)
+        done += step
         torch.cuda.synchronize(dev)
         trace_l.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
-    lrn.log = None
+    plain(log_sweeps, True)
     xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev)
-    losses = net.train(xu, xv, y, epochs=3, seed=seed)
+    tuples = int(y.shape[0])
+    losses = net.train(xu, xv, y, steps=train_steps, seed=seed)
     proposals = prepare(lrn, net)
     torch.cuda.synchronize(dev)
     train_seconds = time.perf_counter() - t0
+    last_train = train_seconds
+    retrains = 0
     while time.perf_counter() - t0 < seconds:
+        if retrain_seconds > 0 and time.perf_counter() - t0 - last_train >= retrain_seconds:
+            log.reset()
+            plain(log_sweeps, True)
+            xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev)
+            tuples += int(y.shape[0])
+            losses += net.train(xu, xv, y, steps=retrain_steps, seed=seed + retrains + 1)
+            proposals = prepare(lrn, net)
+            torch.cuda.synchronize(dev)
+            last_train = time.perf_counter() - t0
+            retrains += 1
         run_learned(lrn, proposals, chunk, steps, ladder)
+        sweeps += chunk
         torch.cuda.synchronize(dev)
         trace_l.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
     lrn.validate(range(0, replicas, max(1, replicas // 64)))
-    result["learned"] = {"trace": trace_l, "sweeps": train_sweeps + chunk * (len(trace_l) - (train_sweeps + chunk - 1)
-                                                                             // chunk),
-                         "training_tuples": int(y.shape[0]), "training_losses": losses,
-                         "seconds_until_learned_proposals": train_seconds,
+    result["learned"] = {"trace": trace_l, "sweeps": sweeps, "training_tuples": tuples, "training_losses": losses,
+                         "retrains": retrains, "seconds_until_learned_proposals": train_seconds,
                          "threshold_seconds": threshold_times(trace_l, thresholds)}
     result["config"] = {"n": n, "replicas": replicas, "ladder": ladder, "seconds": seconds, "seed": seed,
-                        "state_seed": state_seed, "tmin": tmin, "tmax": tmax, "train_sweeps": train_sweeps}
+                        "state_seed": state_seed if start is None else None, "tmin": tmin, "tmax": tmax,
+                        "train_sweeps": train_sweeps, "log_sweeps": log_sweeps, "train_steps": train_steps,
+                        "retrain_seconds": retrain_seconds, "retrain_steps": retrain_steps}
     return result

@@ -120,3 +120,29 @@ def check_columns(columns: np.ndarray) -> None:
     np.add.at(degree, (np.repeat(np.arange(r), m), columns.reshape(-1)), 1)
     if np.any(degree != 2):
         raise ValueError("points are not a row/column 2-factor")
+
+
+def random_two_switches(columns: np.ndarray, moves: int, seed: int) -> np.ndarray:
+    """Copies of interleaved states int32[R][2n] after `moves` random legal
+    2-switches each (the tempering walk's move: swap the columns of two points
+    in different rows so that every row keeps two distinct columns; the column
+    counts are unchanged).  Seeded NumPy draws; illegal draws are redrawn."""
+    cols = np.array(columns, dtype=np.int32, copy=True)
+    if cols.ndim != 2 or cols.shape[1] % 2 or moves < 0:
+        raise ValueError("columns must be int32[R][2n] and moves non-negative")
+    check_columns(cols)
+    r, m = cols.shape
+    rng = np.random.default_rng(seed)
+    rows = np.arange(r)
+    for _ in range(moves):
+        pending = rows.copy()
+        while len(pending):
+            u = rng.integers(0, m, len(pending))
+            v = rng.integers(0, m, len(pending))
+            cu, cv = cols[pending, u], cols[pending, v]
+            ok = ((u >> 1) != (v >> 1)) & (cu != cv) & (cols[pending, u ^ 1] != cv) & (cols[pending, v ^ 1] != cu)
+            done = pending[ok]
+            cols[done, u[ok]] = cv[ok]
+            cols[done, v[ok]] = cu[ok]
+            pending = pending[~ok]
+    return cols

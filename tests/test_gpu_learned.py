@@ -72,9 +72,10 @@ def test_oracle_gradient_is_the_loss_derivative(oracle):
 def test_gradient_matches_float64(pkg, oracle, dev):
     rng = np.random.default_rng(3)
     net = _net(pkg, dev, rng)
-    for pairs in (7, 64, 1000):
+    for pairs in (7, 64, 1000, 999):
         xu, xv = _features(pkg, rng, pairs), _features(pkg, rng, pairs)
-        y = (rng.random(pairs) < 0.3).astype(np.float32)
+        # 0/1 outcomes, and soft Metropolis-probability labels (last case)
+        y = (rng.random(pairs) < 0.3).astype(np.float32) if pairs != 999 else rng.random(pairs).astype(np.float32)
         grad, loss = net.gradient(torch.from_numpy(xu.view(np.int16)).to(dev),
                                   torch.from_numpy(xv.view(np.int16)).to(dev), torch.from_numpy(y).to(dev))
         ref, ref_loss = oracle.mlp_pair_grad(net.params.cpu().numpy().view(np.uint16), xu, xv, y)
@@ -134,16 +135,38 @@ def test_learned_walk_matches_replay(pkg, oracle, dev, n, replicas, ladder, swee
 
 def test_training_on_logged_proposals(pkg, dev):
     chains = pkg.seeded_chains(40, 512, 64, seed=5)
-    log = pkg.learned.ProposalLog(1 << 18, dev)
-    chains.log = log
-    chains.run(30, 80, 64)
+    chains.run(5, 80, 64)
     torch.cuda.synchronize()
+    valid0, acc0 = (int(x) for x in chains.proposals.cpu())
+    log = pkg.learned.ProposalLog(4 * 80 * 512, dev)
+    chains.log = log
+    chains.run(4, 80, 64)
+    chains.log = None
+    torch.cuda.synchronize()
+    valid1, acc1 = (int(x) for x in chains.proposals.cpu())
     rows = log.rows()
-    assert rows.shape[0] == min(int(log.count.item()), log.capacity) > 10000
     r = rows.cpu().numpy()
+    # every valid proposal of the logged sweeps is logged exactly once
+    assert int(log.count.item()) == 4 * 80 * 512
+    assert len(r) == valid1 - valid0 > 10000
+    assert int(r[:, 5].sum()) == acc1 - acc0
     assert set(np.unique(r[:, 5])) <= {0, 1} and np.all(r[r[:, 4] <= 0, 5] == 1)
+    assert np.bincount(r[:, 3], minlength=512).max() <= 4 * 80 and r[:, 2].min() >= 0
+    # a log too small for every replica's reservation keeps whole sweeps only
+    small = pkg.learned.ProposalLog(80 * 100 + 7, dev)
+    chains.log = small
+    chains.run(1, 80, 64)
+    chains.log = None
+    torch.cuda.synchronize()
+    part = small.rows().cpu().numpy()
+    assert int(small.count.item()) == 80 * 512 and 0 < len(part) <= 80 * 100
+    assert np.all(small.entries[80 * 100:].cpu().numpy() == -1)
     net = pkg.learned.ProposalNetwork(seed=1, device=dev)
     xu, xv, y = pkg.learned.training_tensors(rows, chains.temperatures_host, dev)
+    # device features equal the host formula up to one bf16 rounding step
+    host = pkg.learned.point_features(r[:, 0], np.log(chains.temperatures_host.astype(np.float32))[r[:, 3]], r[:, 2])
+    dev_f = pkg.learned.from_bf16(xu.cpu().numpy().view(np.uint16))
+    assert np.allclose(dev_f, pkg.learned.from_bf16(pkg.learned.to_bf16(host)), rtol=2 ** -7, atol=1e-6)
     losses = net.train(xu, xv, y, epochs=4)
     assert losses[-1] < losses[0]
     props = pkg.learned.prepare(chains, net)

@@ -118,3 +118,15 @@ def test_apply_three_cycle(pkg):
     moved.validate()
     before, after = set(state.points()), set(moved.points())
     assert len(before - after) == 3 and len(after - before) == 3
+
+
+def test_random_two_switches_keep_the_two_factor(pkg):
+    P, Q = np.arange(9), np.roll(np.arange(9), 1)
+    cols = np.repeat(np.stack((P, Q), 1).reshape(1, 18), 5, axis=0).astype(np.int32)
+    out = pkg.random_two_switches(cols, 7, seed=3)
+    pkg.state.check_columns(out)
+    assert out.shape == cols.shape and not np.array_equal(out, cols)
+    # every move changes the columns of exactly two slots, so at most 14 differ
+    assert np.all((out != cols).sum(1) <= 14)
+    assert np.array_equal(out, pkg.random_two_switches(cols, 7, seed=3))
+    assert np.array_equal(pkg.random_two_switches(cols, 0, seed=3), cols)

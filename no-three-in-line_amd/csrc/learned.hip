@@ -101,6 +101,19 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     }
     s1 = wave_sum_dpp(part);
   }
+  // inclusive prefix sums of w in slot order (the inverse-CDF table of the
+  // proposal draw); they change only when a move is accepted
+  uint32_t cdf[CH];
+  auto build_cdf = [&]() {
+    uint32_t carry = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint32_t x = wave_scan_dpp((uint32_t)w[c]);
+      cdf[c] = carry + x;
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+    }
+  };
+  build_cdf();
 
   for (int step = 0; step < a.steps; ++step) {
     uint32_t r[4];
@@ -112,14 +125,8 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
       // inverse CDF of w over the slots in slot order
       const uint32_t target = (uint32_t)(((uint64_t)r[1] * (uint64_t)s1) >> 32);
       int below = 0;
-      uint32_t carry = 0;
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const uint32_t x = wave_scan_dpp((uint32_t)w[c]);
-        const int k = lane + kWave * c;
-        below += __popcll(__ballot(k < m && carry + x <= target));
-        carry += (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
-      }
+      for (int c = 0; c < CH; ++c) below += __popcll(__ballot(lane + kWave * c < m && cdf[c] <= target));
       u = below < m - 1 ? below : m - 1;
     }
     const int vv = (int)mulhi32(r[2], (uint32_t)(m - 1));
@@ -187,6 +194,9 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
     const int d = delta < -L.k64 ? -L.k64 : (delta > L.k64 - 1 ? L.k64 - 1 : delta);
     const unsigned long long t64 = register_table64(thr_lo, thr_hi, thr, d + L.k64);
+    // thr64 = 0: r N S1' < 0 is false whatever the weights, so the move is
+    // rejected without reading the moved state's weight row
+    if (t64 == 0ull) continue;
     const uint32_t row2 = weight_row(score + delta);
     const uint64_t wu = (uint32_t)slot_value<CH>(w, u), wv = (uint32_t)slot_value<CH>(w, v);
     const uint64_t big_n = 2ull * s1 + 4ull * (uint64_t)m * (wu + wv);
@@ -250,6 +260,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
       if (k == v) key[c] = key_v2;
     }
     s1 = s1_new;
+    build_cdf();
     score += delta;
     removed += dd;
     accepted += 1;

@@ -27,7 +27,7 @@ from .moves import apply_three_cycle, legal_three_cycle_candidates
 from .state import PairPermutations, points_to_permutations, random_state, random_two_switches
 from .tempering import TemperingChains
 from .batched import BatchGroup, run_batched, seeded_chains
-from . import autotune, learned
+from . import autotune, io, learned
 
 __all__ = [
     "SCORE_AUTO", "SCORE_DETERMINANT", "SCORE_SLOPE", "SENTINEL32", "SENTINEL64",
@@ -35,5 +35,5 @@ __all__ = [
     "delta_table", "delta_table_device", "evaluate_three_cycles", "evaluate_transpositions",
     "apply_three_cycle", "legal_three_cycle_candidates", "evaluate_two_switches", "score_two_switches_gpu",
     "PairPermutations", "points_to_permutations", "random_state", "random_two_switches", "TemperingChains", "_lib",
-    "BatchGroup", "run_batched", "seeded_chains", "autotune", "learned",
+    "BatchGroup", "run_batched", "seeded_chains", "autotune", "io", "learned",
 ]

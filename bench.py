@@ -211,6 +211,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-temper", action="store_true", help="skip the tempering, batched and learned legs")
+    ap.add_argument("--legs", default="temper,batched,learned,near",
+                    help="comma list of the extra legs to run: temper, batched, learned, near")
     ap.add_argument("--ab-seconds", type=float, default=20.0, help="wall-clock budget of each A/B side")
     ap.add_argument("--near-seconds", type=float, default=8.0,
                     help="wall-clock budget of each side of the near-solution A/B")
@@ -274,13 +276,21 @@ def main() -> int:
                 "triple_tests_per_s": POPULATION * (2 * N) * (2 * N - 1) * (2 * N - 2) / 6 / (kernel_ms * 1e-3)}
 
     extra = {}
-    if not args.no_temper:
+    legs = set() if args.no_temper else {x.strip() for x in args.legs.split(",") if x.strip()}
+    unknown = legs - {"temper", "batched", "learned", "near"}
+    if unknown:
+        log(f"bench: unknown legs {sorted(unknown)}")
+        return 2
+    if "temper" in legs:
         log("bench: tempering rate")
         extra["tempering_n75_64x64"] = tempering_rate(pkg, torch)
+    if "batched" in legs:
         log("bench: batched uneven sizes")
         extra["batched_uneven"] = batched_rate(pkg, torch)
+    if "learned" in legs:
         log("bench: learned vs uniform proposals")
         extra["learned_vs_uniform"] = learned_ab(pkg, args.ab_seconds)
+    if "near" in legs:
         log("bench: learned vs uniform proposals from a perturbed near-solution")
         near = learned_ab_near_solution(pkg, args.near_seconds)
         if near is not None:

@@ -241,6 +241,37 @@ int n3_metropolis_thresholds(const float* host_temperatures, int replicas, int k
 int n3_exchange_thresholds(const float* host_temperatures, int replicas, int ladder, int k, uint32_t* host_out,
                            int8_t* host_sign);
 
+/* ---- device search engines (GA / beam) over the scorer ------------------ */
+
+/* Kernel mutate_population (gpu_device_search.py:27-113): child c < elite_count
+ * copies parent c; every other child copies a parent drawn from the first
+ * quality_count parents (probability 160/256) or from all elite_count, then
+ * applies `strength` random legal 2-switches, strength = 1 + [0,
+ * active_strength) or, one time in 32, 1 + [0, max_strength).  Each switch
+ * tries up to 32 draws of (u, v); u is drawn in proportion to the parent's
+ * badness int32[elite_count][2n] with probability bad_bias/100, uniformly
+ * otherwise, and v uniformly.  Draws are Philox keyed by `seed` with counter
+ * (child, generation, draw, "MUTA") in place of the reference's xorshift64*.
+ * Parents and children are P/Q arrays int32[count][n]. */
+int n3_mutate_population(const int32_t* parent_p, const int32_t* parent_q, const int32_t* parent_badness,
+                         int elite_count, int quality_count, int n, int64_t population, int active_strength,
+                         int max_strength, int bad_bias, uint64_t seed, int generation, int32_t* child_p,
+                         int32_t* child_q, void* stream);
+
+/* Kernel enumerate_switches (gpu_exhaustive_beam.py:17-57): child i copies
+ * parent descriptors[i][0] and applies the 2-switch of slots
+ * (descriptors[i][1], descriptors[i][2]) when it is legal; valid[i] says
+ * whether it was.  A descriptor out of range is an error (N3_EINVAL);
+ * synchronises `stream`. */
+int n3_enumerate_switches(const int32_t* parent_p, const int32_t* parent_q, int parent_count,
+                          const int32_t* descriptors, int64_t count, int n, int32_t* child_p, int32_t* child_q,
+                          uint8_t* valid, void* stream);
+
+/* Secondary keys of rank_elites (gpu_device_search.py:185-197) for badness
+ * int32[count][2n]: the maximum and the sum of squares of each row. */
+int n3_elite_keys(const int32_t* badness, int64_t count, int n, int32_t* max_badness, int64_t* sum_squares,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,10 @@ from .moves import apply_three_cycle, legal_three_cycle_candidates
 from .state import PairPermutations, points_to_permutations, random_state, random_two_switches
 from .tempering import TemperingChains
 from .batched import BatchGroup, run_batched, seeded_chains
-from . import autotune, io, learned
+from . import autotune, gpu_device_search, gpu_exhaustive_beam, io, learned
+from .gpu_device_search import (GpuSearchResult, gpu_device_beam_search, mutate_population_device, rank_elites,
+                                rank_elites_device)
+from .gpu_exhaustive_beam import candidate_descriptors, gpu_exhaustive_beam_search
 
 __all__ = [
     "SCORE_AUTO", "SCORE_DETERMINANT", "SCORE_SLOPE", "SENTINEL32", "SENTINEL64",
@@ -36,4 +39,6 @@ __all__ = [
     "apply_three_cycle", "legal_three_cycle_candidates", "evaluate_two_switches", "score_two_switches_gpu",
     "PairPermutations", "points_to_permutations", "random_state", "random_two_switches", "TemperingChains", "_lib",
     "BatchGroup", "run_batched", "seeded_chains", "autotune", "io", "learned",
+    "gpu_device_search", "gpu_exhaustive_beam", "GpuSearchResult", "gpu_device_beam_search", "mutate_population_device",
+    "rank_elites", "rank_elites_device", "candidate_descriptors", "gpu_exhaustive_beam_search",
 ]

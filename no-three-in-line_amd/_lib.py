@@ -92,6 +92,9 @@ SIGNATURES = {
     "n3_evaluate_transpositions": (_i, [_vp, _vp, _vp, _i, _i, _i, _i64, _vp, _vp]),
     "n3_delta_table": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp]),
     "n3_evaluate_three_cycles": (_i, [_vp, _vp, _vp, _i, _vp, _i64, _i64, _vp, _vp]),
+    "n3_mutate_population": (_i, [_vp, _vp, _vp, _i, _i, _i, _i64, _i, _i, _i, _u64, _i, _vp, _vp, _vp]),
+    "n3_enumerate_switches": (_i, [_vp, _vp, _i, _vp, _i64, _i, _vp, _vp, _vp, _vp]),
+    "n3_elite_keys": (_i, [_vp, _i64, _i, _vp, _vp, _vp]),
     "n3_temper_advance": (_i, [ctypes.POINTER(Chains), _i, _i, _u64, _vp]),
     "n3_temper_exchange": (_i, [ctypes.POINTER(Chains), _i, _u64, _vp]),
     "n3_temper_run": (_i, [ctypes.POINTER(Chains), _i, _i, _i, _i, _u64, _vp]),

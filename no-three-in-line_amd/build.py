@@ -18,7 +18,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libno3.so"
 OBJ = HERE / "_build"
-SOURCES = ["runtime.hip", "score.hip", "delta.hip", "temper.hip", "mlp.hip", "learned.hip"]
+SOURCES = ["runtime.hip", "score.hip", "delta.hip", "temper.hip", "mlp.hip", "learned.hip", "search.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

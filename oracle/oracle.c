@@ -17,6 +17,8 @@
  *   n3o_evaluate_three_cycles    accelerated.py:198-276
  *   n3o_walk (tempering replay)  tempering.py:48-110    (walk kernel step semantics)
  *   n3o_exchange                 tempering.py:169-191   (adjacent-temperature swaps)
+ *   n3o_mutate_population        gpu_device_search.py:27-113 (GA children)
+ *   n3o_enumerate_switches       gpu_exhaustive_beam.py:17-57 (enumerated 2-switch children)
  *   n3o_advance_learned          the walk above with learned first-slot proposals
  *                                and the Metropolis-Hastings test (no reference
  *                                counterpart; the rule is stated at the function)
@@ -829,4 +831,95 @@ int64_t n3o_advance_learned(int n, int replicas, int steps, uint64_t seed, uint3
                                    thr64 + (int64_t)r * 2 * k64, k64);
     }
     return valid;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Device search engines: mutate_population (gpu_device_search.py:27-113)    */
+/* and enumerate_switches (gpu_exhaustive_beam.py:17-57), restated slot by   */
+/* slot with the Philox draws (child, generation, draw, "MUTA").             */
+/* ------------------------------------------------------------------------- */
+
+#define TAG_MUTATE 0x4D555441u
+
+static void mutate_draw(uint64_t seed, uint32_t child, uint32_t generation, uint32_t draw, uint32_t out[4])
+{
+    const uint32_t ctr[4] = {child, generation, draw, TAG_MUTATE};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    n3o_philox4x32_10(ctr, key, out);
+}
+
+/* gpu_exhaustive_beam.py:39-54: swap the columns of slots u and v (slot s is
+ * P[s>>1] for even s, Q[s>>1] for odd s) unless the rows coincide, the
+ * columns coincide, or a row would hold one column twice. */
+static int apply_switch(int32_t* p, int32_t* q, int u, int v)
+{
+    const int row_u = u >> 1, row_v = v >> 1;
+    if (row_u == row_v) return 0;
+    int32_t* slot_u = (u & 1) ? q + row_u : p + row_u;
+    int32_t* slot_v = (v & 1) ? q + row_v : p + row_v;
+    const int32_t column_u = *slot_u, column_v = *slot_v;
+    if (column_u == column_v) return 0;
+    const int32_t other_u = (u & 1) ? p[row_u] : q[row_u];
+    const int32_t other_v = (v & 1) ? p[row_v] : q[row_v];
+    if (column_v == other_u || column_u == other_v) return 0;
+    *slot_u = column_v;
+    *slot_v = column_u;
+    return 1;
+}
+
+void n3o_mutate_population(const int32_t* parent_p, const int32_t* parent_q, const int32_t* parent_badness,
+                           int elite_count, int quality_count, int n, int64_t population, int active_strength,
+                           int max_strength, int bad_bias, uint64_t seed, int generation, int32_t* child_p,
+                           int32_t* child_q)
+{
+    const int m = 2 * n;
+    for (int64_t child = 0; child < population; ++child) {
+        uint32_t a[4];
+        mutate_draw(seed, (uint32_t)child, (uint32_t)generation, 0u, a);
+        int parent = (int)child;
+        if (child >= elite_count) {
+            const int pool = (a[0] >> 24) < 160u ? quality_count : elite_count;
+            parent = (int)mulhi32(a[1], (uint32_t)pool);
+        }
+        int32_t* cp = child_p + child * n;
+        int32_t* cq = child_q + child * n;
+        memcpy(cp, parent_p + (int64_t)parent * n, sizeof(int32_t) * n);
+        memcpy(cq, parent_q + (int64_t)parent * n, sizeof(int32_t) * n);
+        if (child < elite_count) continue;
+        const int span = mulhi32(a[2], 32u) == 0u ? max_strength : active_strength;
+        const int strength = 1 + (int)mulhi32(a[3], (uint32_t)span);
+        const int32_t* bad = parent_badness + (int64_t)parent * m;
+        uint32_t total = 0;
+        for (int e = 0; e < m; ++e) total += (uint32_t)bad[e];
+        for (int step = 0; step < strength; ++step) {
+            for (int attempt = 0; attempt < 32; ++attempt) {
+                uint32_t b[4];
+                mutate_draw(seed, (uint32_t)child, (uint32_t)generation, 1u + (uint32_t)(step * 32 + attempt), b);
+                int u = (int)mulhi32(b[2], (uint32_t)m);
+                if ((int)mulhi32(b[0], 100u) < bad_bias && total > 0u) {
+                    /* gpu_device_search.py:84-88: first slot whose running
+                     * badness sum exceeds the threshold */
+                    int64_t threshold = (int64_t)mulhi32(b[1], total);
+                    u = 0;
+                    for (; u < m - 1; ++u) {
+                        threshold -= bad[u];
+                        if (threshold < 0) break;
+                    }
+                }
+                const int v = (int)mulhi32(b[3], (uint32_t)m);
+                if (apply_switch(cp, cq, u, v)) break;
+            }
+        }
+    }
+}
+
+void n3o_enumerate_switches(const int32_t* parent_p, const int32_t* parent_q, const int32_t* descriptors,
+                            int64_t count, int n, int32_t* child_p, int32_t* child_q, uint8_t* valid)
+{
+    for (int64_t i = 0; i < count; ++i) {
+        const int parent = descriptors[3 * i], u = descriptors[3 * i + 1], v = descriptors[3 * i + 2];
+        memcpy(child_p + i * n, parent_p + (int64_t)parent * n, sizeof(int32_t) * n);
+        memcpy(child_q + i * n, parent_q + (int64_t)parent * n, sizeof(int32_t) * n);
+        valid[i] = (uint8_t)apply_switch(child_p + i * n, child_q + i * n, u, v);
+    }
 }

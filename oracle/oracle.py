@@ -76,6 +76,12 @@ def lib():
             ctypes.c_int, ctypes.c_int,
         ]
         L.n3o_advance_learned.restype = ctypes.c_int64
+        _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+        L.n3o_mutate_population.argtypes = [
+            _i32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _i32p, _i32p,
+        ]
+        L.n3o_enumerate_switches.argtypes = [_i32p, _i32p, _i32p, ctypes.c_int64, ctypes.c_int, _i32p, _i32p, _u8p]
         _lib = L
     return _lib
 
@@ -150,6 +156,33 @@ def evaluate_three_cycles(p, q, candidates, current_score: int, badness) -> np.n
     lib().n3o_evaluate_three_cycles(p, q, len(p), cand, len(cand), current_score,
                                     np.ascontiguousarray(badness, dtype=np.int64), out)
     return out
+
+
+def mutate_population(parent_p, parent_q, parent_badness, *, quality_count: int, population: int,
+                      active_strength: int, max_strength: int, bad_bias: int, seed: int, generation: int):
+    """gpu_device_search.py:27-113 with the Philox draws (child, generation,
+    draw, "MUTA"): children P/Q int32[population][n]."""
+    p, q = _i32(parent_p), _i32(parent_q)
+    elite_count, n = p.shape
+    bad = _i32(parent_badness).reshape(elite_count, 2 * n)
+    cp = np.empty((population, n), dtype=np.int32)
+    cq = np.empty((population, n), dtype=np.int32)
+    lib().n3o_mutate_population(p, q, bad, elite_count, quality_count, n, population, active_strength, max_strength,
+                                bad_bias, seed & (2**64 - 1), generation, cp, cq)
+    return cp, cq
+
+
+def enumerate_switches(parent_p, parent_q, descriptors):
+    """gpu_exhaustive_beam.py:17-57: children and validity of the
+    descriptors (parent, u, v)."""
+    p, q = _i32(parent_p), _i32(parent_q)
+    n = p.shape[1]
+    d = _i32(descriptors).reshape(-1, 3)
+    cp = np.empty((len(d), n), dtype=np.int32)
+    cq = np.empty((len(d), n), dtype=np.int32)
+    valid = np.empty(len(d), dtype=np.uint8)
+    lib().n3o_enumerate_switches(p, q, d, len(d), n, cp, cq, valid)
+    return cp, cq, valid.astype(bool)
 
 
 def delta_table(p_batch, q_batch) -> np.ndarray:

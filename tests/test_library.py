@@ -130,3 +130,44 @@ def test_random_two_switches_keep_the_two_factor(pkg):
     assert np.all((out != cols).sum(1) <= 14)
     assert np.array_equal(out, pkg.random_two_switches(cols, 7, seed=3))
     assert np.array_equal(pkg.random_two_switches(cols, 0, seed=3), cols)
+
+
+def _reference_descriptors(badness, global_anchors=False):
+    """The nested loops of gpu_exhaustive_beam.py:67-82, restated."""
+    out = []
+    for parent in range(badness.shape[0]):
+        anchors = list(range(badness.shape[1])) if global_anchors else list(map(int, np.flatnonzero(badness[parent])))
+        anchor_set = set(anchors)
+        for u in anchors:
+            for v in range(badness.shape[1]):
+                if v == u or (v >> 1) == (u >> 1):
+                    continue
+                if v in anchor_set and v < u:
+                    continue
+                out.append((parent, u, v))
+    return np.asarray(out, dtype=np.int32).reshape(-1, 3)
+
+
+def test_candidate_descriptors_follow_the_reference_order(pkg):
+    rng = np.random.default_rng(4)
+    for m in (4, 8, 30):
+        bad = np.where(rng.random((5, m)) < 0.3, rng.integers(1, 5, (5, m)), 0)
+        bad[2] = 0
+        for global_anchors in (False, True):
+            got = pkg.candidate_descriptors(bad, global_anchors=global_anchors)
+            assert np.array_equal(got, _reference_descriptors(bad, global_anchors))
+    # every unordered pair of different rows once, per parent
+    full = pkg.candidate_descriptors(np.zeros((1, 10)), global_anchors=True)
+    assert len(full) == 10 * 9 // 2 - 5
+    assert len(pkg.candidate_descriptors(np.zeros((3, 10)))) == 0
+
+
+def test_rank_elites_is_the_lexicographic_key_order(pkg):
+    rng = np.random.default_rng(8)
+    scores = rng.integers(0, 4, 200)
+    bad = rng.integers(0, 3, (200, 12))
+    order = pkg.rank_elites(scores, bad)
+    keys = [pkg.gpu_device_search.elite_key(int(scores[i]), bad[i]) for i in range(200)]
+    expected = sorted(range(200), key=lambda i: keys[i])  # stable: ties keep index order
+    assert order.dtype == np.int32 and list(order) == expected
+    assert len(pkg.rank_elites(np.empty(0), np.empty((0, 4)))) == 0

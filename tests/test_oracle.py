@@ -365,3 +365,74 @@ def test_learned_early_rejection_never_rejects_an_accepted_move():
             fired32 += 1
             assert exact_reject
     assert fired > 500 and fired32 > 500
+
+
+def _interleave(p, q):
+    out = np.empty((p.shape[0], 2 * p.shape[1]), dtype=np.int32)
+    out[:, 0::2] = p
+    out[:, 1::2] = q
+    return out
+
+
+def test_mutation_keeps_the_two_factor_and_the_elites(oracle, pkg):
+    """Invariants of mutate_population (gpu_device_search.py:27-113): every
+    child is a row/column 2-factor, children below elite_count are exact
+    copies, every other child is within `strength` legal 2-switches of some
+    parent (at most 2 slots per switch), and a strength-1 child differs from
+    its nearest parent by exactly one switch whenever it differs at all."""
+    n, elites = 9, 6
+    p, q = oracle.generate_states(3, n, elites)
+    bad = oracle.score_population(p, q)[1].astype(np.int32)
+    for strength, bias in ((1, 0), (3, 85), (5, 100)):
+        cp, cq = oracle.mutate_population(p, q, bad, quality_count=2, population=300, active_strength=strength,
+                                          max_strength=strength, bad_bias=bias, seed=11, generation=4)
+        kids, parents = _interleave(cp, cq), _interleave(p, q)
+        pkg.state.check_columns(kids)
+        assert np.array_equal(kids[:elites], parents)
+        dist = (kids[elites:, None, :] != parents[None, :, :]).sum(axis=2).min(axis=1)
+        assert dist.max() <= 2 * strength
+        if strength == 1:
+            assert set(np.unique(dist)) <= {0, 2} and (dist == 2).mean() > 0.9
+        again = oracle.mutate_population(p, q, bad, quality_count=2, population=300, active_strength=strength,
+                                         max_strength=strength, bad_bias=bias, seed=11, generation=4)
+        assert np.array_equal(again[0], cp) and np.array_equal(again[1], cq)
+
+
+def test_mutation_bias_moves_bad_points(oracle):
+    """With bad_bias = 100 the first slot of every switch is drawn in
+    proportion to the parent's badness: a strength-1 child of a parent whose
+    only bad slots are known changes one of them (or its partner slot v)."""
+    n = 12
+    p, q = oracle.generate_states(5, n, 1)
+    bad = np.zeros((1, 2 * n), dtype=np.int32)
+    bad[0, [3, 10]] = 1
+    cp, cq = oracle.mutate_population(p, q, bad, quality_count=1, population=400, active_strength=1,
+                                      max_strength=1, bad_bias=100, seed=2, generation=1)
+    kids, parent = _interleave(cp, cq)[1:], _interleave(p, q)[0]
+    changed = kids != parent[None, :]
+    moved = changed.any(axis=1)
+    assert moved.mean() > 0.9
+    assert np.all(changed[moved][:, [3, 10]].any(axis=1))
+
+
+def test_enumeration_matches_the_switch_rule(oracle, pkg):
+    """enumerate_switches (gpu_exhaustive_beam.py:17-57): a legal descriptor
+    changes exactly slots u and v by swapping their columns, and the legal
+    set is the one whose 2-switch score the delta table defines (entries
+    below the sentinel) minus the same-column switches, which
+    accelerated.py:146-150 score as no-ops and enumerate_switches rejects."""
+    n = 8
+    p, q = oracle.generate_states(9, n, 3)
+    desc = np.array([(s, u, v) for s in range(3) for u in range(2 * n) for v in range(2 * n)], dtype=np.int32)
+    cp, cq, valid = oracle.enumerate_switches(p, q, desc)
+    kids, parents = _interleave(cp, cq), _interleave(p, q)
+    table = oracle.delta_table(p, q)
+    for i, (s, u, v) in enumerate(desc):
+        assert valid[i] == (table[s, u, v] != 1 << 30 and parents[s, u] != parents[s, v])
+        if valid[i]:
+            expect = parents[s].copy()
+            expect[u], expect[v] = parents[s, v], parents[s, u]
+            assert np.array_equal(kids[i], expect)
+            assert oracle.score_population(cp[i:i + 1], cq[i:i + 1])[0][0] == table[s, u, v]
+        else:
+            assert np.array_equal(kids[i], parents[s])

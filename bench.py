@@ -190,6 +190,29 @@ def learned_ab_near_solution(pkg, seconds: float, moves: int = 12) -> dict | Non
     return out
 
 
+def search_rates(pkg) -> dict:
+    """The device GA (population 65,536 children per generation, 256 elites)
+    and the exhaustive 2-switch beam (32 elites) from one seeded n=75 state:
+    children scored per second and the best energy reached."""
+    from random import Random
+
+    state = pkg.random_state(75, Random(7))
+    pkg.gpu_device_beam_search(state, seed=1, generations=2, population=4096, elite_size=64, max_strength=8,
+                               energy_window=2)  # warm-up: code objects, allocator
+    ga = pkg.gpu_device_beam_search(state, seed=1, generations=20, population=65536, elite_size=256,
+                                    max_strength=8, energy_window=2)
+    beam = pkg.gpu_exhaustive_beam_search(state, seed=1, generations=5, elite_size=32, energy_window=1)
+    return {
+        "ga": {"n": 75, "population": 65536, "elite_size": 256, "generations": ga.generations,
+               "seconds": ga.elapsed_seconds, "children_per_s": ga.evaluations / ga.elapsed_seconds,
+               "best_E": ga.score},
+        "exhaustive_beam": {"n": 75, "elite_size": 32, "generations": beam.generations,
+                            "seconds": beam.elapsed_seconds,
+                            "children_scored_per_s": beam.evaluations / beam.elapsed_seconds,
+                            "best_E": beam.score},
+    }
+
+
 def calibrate_copy() -> int:
     """A plain device-to-device copy of 256 MiB, five times: profiled with
     FETCH_SIZE / WRITE_SIZE it gives the counters' reading for known bytes."""
@@ -211,8 +234,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-temper", action="store_true", help="skip the tempering, batched and learned legs")
-    ap.add_argument("--legs", default="temper,batched,learned,near",
-                    help="comma list of the extra legs to run: temper, batched, learned, near")
+    ap.add_argument("--legs", default="temper,batched,learned,near,search",
+                    help="comma list of the extra legs to run: temper, batched, learned, near, search")
     ap.add_argument("--ab-seconds", type=float, default=20.0, help="wall-clock budget of each A/B side")
     ap.add_argument("--near-seconds", type=float, default=8.0,
                     help="wall-clock budget of each side of the near-solution A/B")
@@ -277,7 +300,7 @@ def main() -> int:
 
     extra = {}
     legs = set() if args.no_temper else {x.strip() for x in args.legs.split(",") if x.strip()}
-    unknown = legs - {"temper", "batched", "learned", "near"}
+    unknown = legs - {"temper", "batched", "learned", "near", "search"}
     if unknown:
         log(f"bench: unknown legs {sorted(unknown)}")
         return 2
@@ -295,6 +318,9 @@ def main() -> int:
         near = learned_ab_near_solution(pkg, args.near_seconds)
         if near is not None:
             extra["learned_vs_uniform_near_solution"] = near
+    if "search" in legs:
+        log("bench: device GA and exhaustive beam")
+        extra["search_engines"] = search_rates(pkg)
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline")

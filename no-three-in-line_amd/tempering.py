@@ -5,13 +5,16 @@ Differences from the reference, all deliberate:
   - random numbers are Philox4x32-10 words keyed by the seed with counter
     (step, sweep, slot) for proposals and (pair, round, ladder) for exchanges,
     so that any trajectory can be replayed on the CPU from (seed, sweep,
-    round); there is no per-replica RNG state (`rngs`) to carry;
+    round); there is no per-replica RNG state to carry (checkpoints still hold
+    the reference's `rngs`, `round` and `host_rng` keys, so either engine can
+    resume the other's files);
   - acceptance compares a 32-bit word with the precomputed integer threshold
     floor(2^32 exp(-x)) (n3_metropolis_thresholds / n3_exchange_thresholds)
     instead of log(U) < -x in double;
   - exchange runs on the device (no host round trip) and, like the
     reference's, swaps the replicas' states between fixed-temperature slots;
-  - umbrella potentials (strengths != 0) are not supported on the device.
+  - umbrella potentials (strengths != 0, --umbrella-max) are not supported
+    on the device and raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -232,20 +235,45 @@ class TemperingChains:
             if len(reference - set((j // 2, int(c)) for j, c in enumerate(row))) != int(value):
                 raise AssertionError("incremental distance mismatch")
 
+    # checkpoint keys of the reference (tempering.py:206-225); `rngs`,
+    # `round` and `host_rng` are its RNG state, which this engine does not
+    # use but writes so that the reference can resume from these files
+    REFERENCE_KEYS = ("columns", "badness", "scores", "rngs", "best_columns", "best_scores", "accepted",
+                      "reference", "removed_points", "centers", "strengths")
+
     def save(self, path) -> None:
+        """tempering.py:206-211: np.savez_compressed with the reference's keys
+        (plus `seed`, `counters` and `proposals`, which resume this engine's
+        Philox streams exactly)."""
         arrays = {name: getattr(self, name).cpu().numpy() for name in (
             "columns", "badness", "scores", "best_columns", "best_scores", "accepted", "reference",
             "removed_points", "centers", "strengths", "counters", "proposals")}
-        np.savez_compressed(path, **arrays, temperatures=self.temperatures_host, seed=np.asarray(self.seed, np.uint64))
+        host = np.random.default_rng([self.seed & (2**63 - 1), self.round])
+        rngs = host.integers(1, 2**64 - 1, size=len(self.temperatures_host), dtype=np.uint64)
+        np.savez_compressed(path, **arrays, rngs=rngs, temperatures=self.temperatures_host,
+                            round=np.asarray(self.round), host_rng=np.asarray(json.dumps(host.bit_generator.state)),
+                            seed=np.asarray(self.seed, np.uint64))
 
     @classmethod
     def restore(cls, path):
+        """tempering.py:213-225: a checkpoint of this engine, or of the
+        reference (which carries no seed or sweep counter: the seed is then
+        taken from its host_rng state and both counters from its round, so a
+        resumed run still draws fresh Philox counters)."""
         with np.load(path, allow_pickle=False) as saved:
-            obj = cls(saved["columns"], saved["temperatures"], int(saved["seed"]))
+            if "seed" in saved:
+                seed = int(saved["seed"])
+            else:
+                state = json.loads(str(saved["host_rng"]))
+                seed = int(state["state"]["state"]) & (2**64 - 1)
+            obj = cls(saved["columns"], saved["temperatures"], seed)
             for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted", "reference",
                          "removed_points", "centers", "strengths", "counters", "proposals"):
                 if name in saved:
                     setattr(obj, name, torch.from_numpy(np.ascontiguousarray(saved[name])).to(obj.device))
+            if "counters" not in saved and "round" in saved:
+                r = int(saved["round"])
+                obj.counters = torch.tensor([r, r, 0, 0], dtype=torch.int32, device=obj.device)
         obj.validate()
         return obj
 
@@ -264,7 +292,11 @@ def main(argv=None) -> int:
     parser.add_argument("--steps", type=int, default=100)
     parser.add_argument("--bad-bias", type=int, default=0)
     parser.add_argument("--resume")
+    parser.add_argument("--umbrella-max", type=float, default=0)
+    parser.add_argument("--umbrella-strength", type=float, default=0.2)
     args = parser.parse_args(argv)
+    if args.umbrella_max:
+        raise NotImplementedError("umbrella potentials (--umbrella-max) are not supported by the device walk")
     output = Path(args.output)
     output.mkdir(parents=True, exist_ok=True)
     states, n = [], None
@@ -285,6 +317,7 @@ def main(argv=None) -> int:
                              dtype=np.int32)
         temperatures = np.tile(np.geomspace(args.tmin, args.tmax, args.ladder_size), args.replicas // args.ladder_size)
         chains = TemperingChains(columns, temperatures, args.seed)
+    reference = set((i // 2, int(c)) for i, c in enumerate(states[0]))
     (output / "parameters.json").write_text(json.dumps(vars(args), indent=2) + "\n", encoding="utf-8")
     start = time.perf_counter()
     best, proposals, exchanges = 10**9, 0, 0
@@ -309,9 +342,12 @@ def main(argv=None) -> int:
                 break
         if epoch % 10 == 0 or epoch == args.rounds:
             chains.validate([epoch % args.replicas])
+            columns = chains.columns.cpu().numpy()
+            distances = [len(reference.symmetric_difference((i // 2, int(c)) for i, c in enumerate(row)))
+                         for row in columns]
             payload = {"epoch": epoch, "proposals": proposals, "elapsed_seconds": time.perf_counter() - start,
-                       "best": best, "scores": chains.scores.cpu().numpy().tolist(), "exchanges": exchanges,
-                       "accepted": int(chains.accepted.cpu().numpy().astype(np.int64).sum())}
+                       "best": best, "scores": chains.scores.cpu().numpy().tolist(), "distances": distances,
+                       "exchanges": exchanges, "accepted": int(chains.accepted.cpu().numpy().astype(np.int64).sum())}
             with (output / "history.jsonl").open("a", encoding="utf-8") as stream:
                 stream.write(json.dumps(payload) + "\n")
         if epoch % 50 == 0:
@@ -319,7 +355,7 @@ def main(argv=None) -> int:
     chains.validate()
     chains.save(output / "checkpoint.npz")
     result = {"best": best, "proposals": proposals, "elapsed_seconds": time.perf_counter() - start,
-              "accepted": int(chains.accepted.cpu().numpy().astype(np.int64).sum())}
+              "accepted": int(chains.accepted.cpu().numpy().astype(np.int64).sum()), "distance_frontier": {}}
     (output / "summary.json").write_text(json.dumps(result, indent=2) + "\n", encoding="utf-8")
     print(json.dumps(result), flush=True)
     return 0

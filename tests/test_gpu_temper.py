@@ -99,3 +99,96 @@ def test_corrupted_state_is_detected(pkg, oracle, dev):
     chains.scores[3] += 1
     with pytest.raises(AssertionError):
         chains.validate()
+
+
+REFERENCE_CHECKPOINT_KEYS = {"columns", "badness", "scores", "rngs", "best_columns", "best_scores", "accepted",
+                             "reference", "removed_points", "centers", "strengths", "temperatures", "round",
+                             "host_rng"}
+
+
+def test_checkpoint_carries_the_reference_schema(pkg, dev, tmp_path):
+    """tempering.py:206-225: a checkpoint of this engine holds every key the
+    reference's restore reads, with its dtypes; the reference's restore steps
+    (np.load without pickles, host_rng JSON into a PCG64 generator) succeed."""
+    import json
+
+    chains = pkg.seeded_chains(20, 64, 16, seed=4)
+    chains.run(7, 40, 16)
+    torch.cuda.synchronize()
+    chains.save(tmp_path / "c.npz")
+    with np.load(tmp_path / "c.npz", allow_pickle=False) as saved:
+        assert REFERENCE_CHECKPOINT_KEYS <= set(saved.files)
+        assert saved["columns"].dtype == np.int32 and saved["columns"].shape == (64, 40)
+        assert saved["rngs"].dtype == np.uint64 and saved["rngs"].shape == (64,) and np.all(saved["rngs"] > 0)
+        assert saved["centers"].dtype == np.float32 and saved["temperatures"].dtype == np.float32
+        assert int(saved["round"]) == chains.round == 7
+        generator = np.random.default_rng(0)
+        generator.bit_generator.state = json.loads(str(saved["host_rng"]))
+        generator.random()
+
+
+def test_restore_reads_a_reference_checkpoint(pkg, oracle, dev, tmp_path):
+    """A checkpoint in the reference's schema only (no seed or counters): the
+    restored chains validate against a full recount, start their Philox
+    counters at the saved round and run on."""
+    import json
+
+    n, replicas = 12, 32
+    p, q = oracle.generate_states(3, n, replicas)
+    columns = np.stack((p, q), axis=2).reshape(replicas, 2 * n).astype(np.int32)
+    scores, badness = oracle.score_population(p, q)
+    host = np.random.default_rng(99)
+    arrays = {
+        "columns": columns, "badness": badness.astype(np.int32), "scores": scores.astype(np.int32),
+        "rngs": host.integers(1, 2**64 - 1, size=replicas, dtype=np.uint64),
+        "best_columns": columns, "best_scores": scores.astype(np.int32),
+        "accepted": np.zeros(replicas, np.int32), "reference": columns[0],
+        "centers": np.zeros(replicas, np.float32), "strengths": np.zeros(replicas, np.float32),
+        "temperatures": np.tile(np.geomspace(0.2, 4.0, 16), 2).astype(np.float32),
+        "round": np.asarray(12), "host_rng": np.asarray(json.dumps(host.bit_generator.state)),
+    }
+    ref = set((i // 2, int(c)) for i, c in enumerate(columns[0]))
+    arrays["removed_points"] = np.array([len(ref - set((i // 2, int(c)) for i, c in enumerate(row)))
+                                         for row in columns], np.int32)
+    np.savez_compressed(tmp_path / "ref.npz", **arrays)
+    chains = pkg.TemperingChains.restore(tmp_path / "ref.npz")
+    assert chains.round == 12 and chains.sweep == 12
+    assert np.array_equal(chains.columns.cpu().numpy(), columns)
+    chains.run(5, 24, 16)
+    torch.cuda.synchronize()
+    chains.validate()
+    assert chains.round == 17
+
+
+def test_driver_writes_the_reference_outputs(pkg, dev, tmp_path):
+    """tempering.main (tempering.py:228-319): parameters.json, best-E*.json
+    candidates, history.jsonl, checkpoint.npz and summary.json with the
+    reference's fields; --resume continues from the checkpoint."""
+    import json
+    from random import Random
+
+    state = pkg.random_state(14, Random(2))
+    initial = tmp_path / "initial.json"
+    pkg.io.write_candidate(initial, 14, state.points())
+    out = tmp_path / "run"
+    args = ["--initial", str(initial), "--output", str(out), "--seed", "5", "--replicas", "32", "--ladder-size", "8",
+            "--rounds", "20", "--steps", "30"]
+    assert pkg.tempering.main(args) == 0
+    assert json.loads((out / "parameters.json").read_text())["seed"] == 5
+    lines = [json.loads(x) for x in (out / "history.jsonl").read_text().splitlines()]
+    assert [x["epoch"] for x in lines] == [10, 20]
+    for x in lines:
+        assert {"epoch", "proposals", "elapsed_seconds", "best", "scores", "distances", "exchanges",
+                "accepted"} <= set(x)
+        assert len(x["scores"]) == len(x["distances"]) == 32
+    summary = json.loads((out / "summary.json").read_text())
+    assert {"best", "proposals", "elapsed_seconds", "accepted", "distance_frontier"} <= set(summary)
+    best = sorted(out.glob("best-E*.json"), key=lambda f: int(f.stem[6:]))[0]
+    n, points, meta = pkg.io.load_candidate(best)
+    assert n == 14 and meta["conflicts"] == summary["best"] == int(best.stem[6:])
+    again = pkg.points_to_permutations(points, 14)
+    assert pkg.score_population_gpu(np.array([again.P]), np.array([again.Q]))[0] == summary["best"]
+    assert pkg.tempering.main(args[:-4] + ["--rounds", "10", "--steps", "30", "--resume",
+                                           str(out / "checkpoint.npz")]) == 0
+    with pytest.raises(NotImplementedError):
+        pkg.tempering.main(args + ["--umbrella-max", "3"])

@@ -97,3 +97,9 @@ def generate_states(seed: int, n: int, count: int, first_index: int = 0,
 def gpu_device_name() -> str:
     dev = _lib.device()
     return torch.cuda.get_device_name(dev)
+
+
+def score_and_badness(p, q) -> tuple[int, np.ndarray]:
+    """accelerated.py:14-37 on the GPU for one state: (score, badness int64[2n])."""
+    scores, badness = score_population_badness(np.asarray(p)[None, :], np.asarray(q)[None, :])
+    return int(scores[0]), badness[0].astype(np.int64)

@@ -85,9 +85,10 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
             "single_thread_states_per_s": single}
 
 
-def tempering_rate(pkg, torch, sweeps: int = 100) -> dict:
-    """64 ladders x 64 temperatures (geometric 0.05..2.0) at n=75, 150
-    proposals per sweep, exchange every sweep, graph-captured."""
+def tempering_rate(pkg, torch, sweeps: int = 10000) -> dict:
+    """Config 3 in full: 64 ladders x 64 temperatures (geometric 0.05..2.0)
+    at n=75, 10,000 sweeps of 150 proposals, exchange every sweep,
+    graph-captured; a sample of replicas is recounted afterwards."""
     import numpy as np
 
     replicas, ladder, steps = 4096, 64, 150

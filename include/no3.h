@@ -206,8 +206,8 @@ int n3_mlp_grad(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, 
 int n3_mlp_adam(float* master, uint16_t* params, const float* grad, float* m, float* v, int step, float lr,
                 float beta1, float beta2, float eps, void* stream);
 
-/* Integer proposal weights table[t][e][b] = clamp(round(4096 exp(clamp(g,
- * -6, 6))), 1, 2^21) of the network's logit g for temperature index t
+/* Integer proposal weights table[t][e][b] = clamp(round(128 exp(clamp(g,
+ * -6, 5.5))), 1, 2^15) of the network's logit g for temperature index t
  * (log_temps[t]), energy e in [0, max_energy] and count b in [0, 63]. */
 int n3_mlp_weight_table(const uint16_t* params, const float* log_temps, int temps, int max_energy, uint32_t* table,
                         void* stream);

@@ -1,23 +1,27 @@
 // Tempering walk with learned move proposals.
 //
-// Proposal: the first slot u comes from the mixture
-//   pi_x(k) = 0.2 / m + 0.8 w_x(k) / S1_x,   S1_x = sum_k w_x(k),
-// where w_x(k) = W[t][E(x)][badness_x(k)] is the proposal network's tabulated
-// integer weight for slot k (temperature index t of the replica slot, energy
-// E(x), the slot's collinear-pair count); the second slot v is uniform over
-// the other m - 1.  The unordered move {u, v} has probability
-//   q(x -> x') = (pi_x(u) + pi_x(v)) / (m - 1),
+// The proposal network scores a move {u, v} by the additive pair logit
+// g(u) + g(v) - b2, so the softmax over all pairs factorises: a pair is drawn
+// with probability proportional to w(u) w(v), where w_x(k) = W[t][E(x)][b]
+// is the network's tabulated integer weight of slot k (temperature index t
+// of the replica slot, energy E(x), the slot's collinear-pair count b),
+// clamped to [1, 2^15].  The proposal mixes 20% uniform pairs with 80% of
+// this softmax: with probability 0.2 both slots are uniform, otherwise both
+// are drawn from w / S1 (S1 = sum_k w(k)) by the inverse CDF; a draw with
+// both slots in one row is an invalid proposal.  The unordered pair {u, v}
+// has probability
+//   q(x -> x') = 2 (0.2 / m^2 + 0.8 w(u) w(v) / S1^2) = 0.4 N / (m^2 S1^2),
+//   N = S1^2 + 4 m^2 w(u) w(v),
 // and the reverse move from x' is the same pair, so the Hastings ratio is
-//   q(x' -> x) / q(x -> x') = (N' S1) / (N S1'),  N = 2 S1 + 4 m (w(u) + w(v))
-// (pi = (S1 + 4 m w) / (5 m S1)).  Acceptance stays on the exact integer delta:
-//   accept  iff  r * N * S1' < thr64(delta) * N' * S1
+// (N' S1^2) / (N S1'^2).  Acceptance stays on the exact integer delta:
+//   accept  iff  r N S1'^2 < thr64(delta) N' S1^2
 // with r the move's 32-bit Philox word and thr64(d) = floor(2^32 exp(-d/T))
-// (saturating at 2^63), evaluated in 128-bit integers -- so a CPU replay with
-// the same weight table reproduces every trajectory bit for bit.  Evaluating
-// the reverse probability needs the badness of every slot in x' (two more
-// direction histograms, from u and v in x); a proposal is rejected before
-// them when an upper bound of the Hastings ratio, from quantities already
-// known, rejects it (the exact test would reject as well).
+// (saturating at 2^63), evaluated exactly (a 128-bit left side against a
+// 192-bit right side) -- so a CPU replay with the same weight table
+// reproduces every trajectory bit for bit.  Evaluating the reverse
+// probability needs the badness of every slot in x' (two more direction
+// histograms, from u and v in x); only a proposal whose acceptance word is 0
+// is rejected before them.
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -42,9 +46,26 @@ struct LearnedArgs {
 // Weight of a slot with collinear-pair count `count` from a weight row held
 // one entry per lane: per-lane counts use a lane permute, uniform counts a
 // scalar read.
-__device__ inline uint32_t weight_of(uint32_t row, int count) { return (uint32_t)__shfl((int)row, count < 63 ? count : 63, kWave); }
+constexpr uint32_t kMaxWeight = 1u << 15;
+
+__device__ inline uint32_t clamp_weight(uint32_t w) { return w < 1u ? 1u : (w > kMaxWeight ? kMaxWeight : w); }
+__device__ inline uint32_t weight_of(uint32_t row, int count) {
+  return (uint32_t)__shfl((int)row, count < 63 ? count : 63, kWave);
+}
 __device__ inline uint32_t weight_at(uint32_t row, int count) {
   return (uint32_t)__builtin_amdgcn_readlane((int)row, count < 63 ? count : 63);
+}
+
+// a < p s for a, p < 2^128 and s < 2^64: p s = p_hi s 2^64 + p_lo s may
+// need 192 bits, so it is compared limb by limb.
+__device__ inline bool less_than_product(unsigned __int128 a, unsigned __int128 p, uint64_t s) {
+  const unsigned __int128 lo = (unsigned __int128)(uint64_t)p * s;
+  const unsigned __int128 hi = (unsigned __int128)(uint64_t)(p >> 64) * s;
+  if ((uint64_t)(hi >> 64) != 0) return true;
+  const unsigned __int128 shifted = hi << 64;
+  const unsigned __int128 sum = shifted + lo;
+  if (sum < shifted) return true;
+  return a < sum;
 }
 
 template <int CH>
@@ -83,7 +104,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
   const unsigned long long thr_hi = lane + kWave < 2 * L.k64 ? thr[lane + kWave] : 0ull;
   auto weight_row = [&](int energy) -> uint32_t {
     const int e = energy < 0 ? 0 : (energy > L.max_energy ? L.max_energy : energy);
-    return wt[(long long)e * 64 + lane];
+    return clamp_weight(wt[(long long)e * 64 + lane]);
   };
   int w[CH];
   uint32_t s1;
@@ -118,19 +139,23 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
   for (int step = 0; step < a.steps; ++step) {
     uint32_t r[4];
     philox4x32_10((uint32_t)step, sweep, (uint32_t)slot, kTagLearn, a.seed_lo, a.seed_hi, r);
-    int u;
+    int u, v;
     if (r[0] < kUniformShare) {
       u = (int)mulhi32(r[1], (uint32_t)m);
+      v = (int)mulhi32(r[2], (uint32_t)m);
     } else {
-      // inverse CDF of w over the slots in slot order
-      const uint32_t target = (uint32_t)(((uint64_t)r[1] * (uint64_t)s1) >> 32);
-      int below = 0;
+      // both slots by the inverse CDF of w over the slots in slot order
+      const uint32_t target_u = mulhi32(r[1], s1), target_v = mulhi32(r[2], s1);
+      int below_u = 0, below_v = 0;
 #pragma unroll
-      for (int c = 0; c < CH; ++c) below += __popcll(__ballot(lane + kWave * c < m && cdf[c] <= target));
-      u = below < m - 1 ? below : m - 1;
+      for (int c = 0; c < CH; ++c) {
+        const bool live = lane + kWave * c < m;
+        below_u += __popcll(__ballot(live && cdf[c] <= target_u));
+        below_v += __popcll(__ballot(live && cdf[c] <= target_v));
+      }
+      u = below_u < m - 1 ? below_u : m - 1;
+      v = below_v < m - 1 ? below_v : m - 1;
     }
-    const int vv = (int)mulhi32(r[2], (uint32_t)(m - 1));
-    const int v = vv + (vv >= u ? 1 : 0);
     const int ru = u >> 1, rv = v >> 1;
     const int key_u = slot_value<CH>(key, u), key_v = slot_value<CH>(key, v);
     const int cu = key_u - ru * width, cv = key_v - rv * width;
@@ -199,25 +224,9 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     if (t64 == 0ull) continue;
     const uint32_t row2 = weight_row(score + delta);
     const uint64_t wu = (uint32_t)slot_value<CH>(w, u), wv = (uint32_t)slot_value<CH>(w, v);
-    const uint64_t big_n = 2ull * s1 + 4ull * (uint64_t)m * (wu + wv);
-    if (t64 < (1ull << 32)) {
-      // Early rejection: with X = w'(u) + w'(v) (known now) and
-      // S1' >= B = X + (m - 2) min(row), N'/S1' = 2 + 4 m X / S1' <= A / B,
-      // A = 2 B + 4 m X, so r N B >= thr S1 A implies r N S1' >= thr N' S1,
-      // i.e. the full test below would reject.  The products (< 2^93) are
-      // formed in fp32: at most 13 roundings in all (conversions, products,
-      // the sum in A, the margin product) bound the error of the ratio of the
-      // two sides by (1 + 2^-24)^13 < 1 + 2^-19, so rejecting only when the
-      // left side exceeds the right by the factor 1 + 2^-13 never rejects a
-      // move the exact test would accept; moves inside the margin go on.
-      const uint32_t x2 = weight_at(row2, bad_u2) + weight_at(row2, bad_v2);
-      const uint32_t lower = x2 + (uint32_t)(m - 2) * wave_min_dpp(row2);
-      const float upper = 2.0f * (float)lower + 4.0f * (float)m * (float)x2;
-      const float lhs = (float)r[3] * (float)big_n * (float)lower;
-      const float rhs = (float)t64 * (float)s1 * upper;
-      if (lhs > rhs * (1.0f + 0x1p-13f)) continue;
-    }
-
+    const uint64_t mm4 = 4ull * (uint64_t)m * (uint64_t)m;
+    const uint64_t s1sq = (uint64_t)s1 * s1;
+    const uint64_t big_n = s1sq + mm4 * wu * wv;  // < 2^48 (w <= 2^15, m <= 200)
     // badness of every slot in the moved state (needed for the reverse move)
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
@@ -246,10 +255,11 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     }
     const uint32_t s1_new = wave_sum_dpp(part2);
     const uint64_t wu2 = (uint32_t)slot_value<CH>(w2, u), wv2 = (uint32_t)slot_value<CH>(w2, v);
-    const uint64_t big_n2 = 2ull * s1_new + 4ull * (uint64_t)m * (wu2 + wv2);
-    const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * s1_new;
-    const unsigned __int128 rhs = (unsigned __int128)t64 * big_n2 * s1;
-    if (!(lhs < rhs)) continue;
+    const uint64_t s1sq_new = (uint64_t)s1_new * s1_new;
+    const uint64_t big_n2 = s1sq_new + mm4 * wu2 * wv2;
+    // r N S1'^2 < 2^126; thr N' < 2^111 times S1^2 < 2^46 may need 192 bits
+    const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * s1sq_new;
+    if (!less_than_product(lhs, (unsigned __int128)t64 * big_n2, s1sq)) continue;
 
 #pragma unroll
     for (int c = 0; c < CH; ++c) {

@@ -281,8 +281,9 @@ __global__ void adam_kernel(float* __restrict__ master, uint16_t* __restrict__ p
 }
 
 // Proposal weights for every (temperature t, energy e, count b):
-// table[(t * (max_energy + 1) + e) * 64 + b] = clamp(round(4096 exp(clamp(g,
-// -6, 6))), 1, 2^21) with g the network's logit.
+// table[(t * (max_energy + 1) + e) * 64 + b] = clamp(round(128 exp(clamp(g,
+// -6, 5.5))), 1, 2^15) with g the network's logit (2^15 keeps the learned
+// walk's exact acceptance test within 128-bit products).
 __global__ void __launch_bounds__(256) weight_table_kernel(const uint16_t* __restrict__ params,
                                                            const float* __restrict__ log_temps, int temps,
                                                            int max_energy, uint32_t* __restrict__ table) {
@@ -308,10 +309,10 @@ __global__ void __launch_bounds__(256) weight_table_kernel(const uint16_t* __res
     for (int i = 0; i < 4; ++i) x[i] = (short)float_to_bf16(f[k0 + i]);
     floatx16_t acc[4];
     hidden_tile(params, x, acc);
-    const float g = fminf(fmaxf(output_row(b1, w2, b2, acc), -6.0f), 6.0f);
+    const float g = fminf(fmaxf(output_row(b1, w2, b2, acc), -6.0f), 5.5f);
     if (lane < 32) {
-      const float w = rintf(4096.0f * expf(g));
-      table[row] = (uint32_t)fminf(fmaxf(w, 1.0f), 2097152.0f);
+      const float w = rintf(128.0f * expf(g));
+      table[row] = (uint32_t)fminf(fmaxf(w, 1.0f), 32768.0f);
     }
   }
 }

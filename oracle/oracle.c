@@ -19,7 +19,7 @@
  *   n3o_exchange                 tempering.py:169-191   (adjacent-temperature swaps)
  *   n3o_mutate_population        gpu_device_search.py:27-113 (GA children)
  *   n3o_enumerate_switches       gpu_exhaustive_beam.py:17-57 (enumerated 2-switch children)
- *   n3o_advance_learned          the walk above with learned first-slot proposals
+ *   n3o_advance_learned          the walk above with learned pair proposals
  *                                and the Metropolis-Hastings test (no reference
  *                                counterpart; the rule is stated at the function)
  *
@@ -691,15 +691,31 @@ void n3o_metropolis_thresholds64(const float* temperatures, int replicas, int k,
     }
 }
 
-/* The walk of walk_slot with the first slot drawn from
- *   pi(k) = 0.2 / m + 0.8 w(k) / S1   (w(k) = wt[E][min(bad[k], 63)])
- * by a 32-bit coin (< floor(0.2 * 2^32): uniform u = mulhi(r1, m); else the
- * first k whose inclusive prefix of w exceeds floor(r1 * S1 / 2^32)), the
- * second uniformly from the other m - 1 (v' = mulhi(r2, m - 1), v = v' + [v'
- * >= u]), and the Metropolis-Hastings test
- *   r3 * N * S1' < thr64(delta) * N' * S1,  N = 2 S1 + 4 m (w(u) + w(v))
- * in 128-bit integers, primes for the moved state.  Philox counter
+/* The walk of walk_slot with learned pair proposals: weights
+ * w(k) = clamp(wt[E][min(bad[k], 63)], 1, 2^15); a 32-bit coin r0 below
+ * floor(0.2 * 2^32) draws both slots uniformly (u = mulhi(r1, m),
+ * v = mulhi(r2, m)), otherwise each slot is the first k whose inclusive
+ * prefix of w exceeds mulhi(r1, S1) (resp. mulhi(r2, S1)); both slots in one
+ * row make the proposal invalid.  The Metropolis-Hastings test is
+ *   r3 * N * S1'^2 < thr64(delta) * N' * S1^2,  N = S1^2 + 4 m^2 w(u) w(v),
+ * evaluated exactly (primes for the moved state).  Philox counter
  * (step, sweep, slot, "LRND"). */
+/* Learned walk weights are clamped to [1, 2^15]. */
+static inline uint32_t clamp_weight(uint32_t w) { return w < 1u ? 1u : (w > 32768u ? 32768u : w); }
+
+/* x * y for x < 2^128, y < 2^64 as three 64-bit limbs (schoolbook), then
+ * a < x * y for a < 2^128. */
+static int below_product(unsigned __int128 a, unsigned __int128 x, uint64_t y)
+{
+    const uint64_t x0 = (uint64_t)x, x1 = (uint64_t)(x >> 64);
+    const unsigned __int128 t0 = (unsigned __int128)x0 * y;
+    const unsigned __int128 t1 = (unsigned __int128)x1 * y + (uint64_t)(t0 >> 64);
+    const uint64_t limb0 = (uint64_t)t0, limb1 = (uint64_t)t1, limb2 = (uint64_t)(t1 >> 64);
+    if (limb2 != 0) return 1;
+    const uint64_t a1 = (uint64_t)(a >> 64), a0 = (uint64_t)a;
+    return a1 < limb1 || (a1 == limb1 && a0 < limb0);
+}
+
 static int64_t walk_slot_learned(int n, int steps, uint64_t seed, uint32_t sweep, uint32_t slot, int32_t* col,
                                  int32_t* bad, int32_t* score_io, int32_t* best_col, int32_t* best_score,
                                  int32_t* accepted, const int32_t* reference, int32_t* removed_io, const uint32_t* wt,
@@ -714,29 +730,36 @@ static int64_t walk_slot_learned(int n, int steps, uint64_t seed, uint32_t sweep
 #define ROW(e) ((int64_t)((e) < 0 ? 0 : ((e) > max_energy ? max_energy : (e))) * 64)
     uint32_t s1 = 0;
     for (int i = 0; i < m; ++i) {
-        w[i] = wt[ROW(score) + (bad[i] < 63 ? bad[i] : 63)];
+        w[i] = clamp_weight(wt[ROW(score) + (bad[i] < 63 ? bad[i] : 63)]);
         s1 += w[i];
     }
     for (int step = 0; step < steps; ++step) {
         uint32_t ctr[4] = {(uint32_t)step, sweep, slot, TAG_LEARN}, r[4];
         n3o_philox4x32_10(ctr, key, r);
-        int u;
+        /* 0.2: a uniform pair; 0.8: both slots drawn in proportion to w (the
+         * first slot whose running weight sum exceeds the target) */
+        int u, v;
         if (r[0] < 858993459u) {
             u = (int)mulhi32(r[1], (uint32_t)m);
+            v = (int)mulhi32(r[2], (uint32_t)m);
         } else {
-            const uint32_t target = (uint32_t)(((uint64_t)r[1] * (uint64_t)s1) >> 32);
+            const uint32_t target_u = mulhi32(r[1], s1), target_v = mulhi32(r[2], s1);
             uint32_t acc = 0;
             u = m - 1;
+            v = m - 1;
+            int found_u = 0, found_v = 0;
             for (int i = 0; i < m; ++i) {
                 acc += w[i];
-                if (acc > target) {
+                if (!found_u && acc > target_u) {
                     u = i;
-                    break;
+                    found_u = 1;
+                }
+                if (!found_v && acc > target_v) {
+                    v = i;
+                    found_v = 1;
                 }
             }
         }
-        const int vv = (int)mulhi32(r[2], (uint32_t)(m - 1));
-        const int v = vv + (vv >= u ? 1 : 0);
         const int32_t cu = col[u], cv = col[v];
         const int valid = score > 0 && (u >> 1) != (v >> 1) && cu != cv && col[u ^ 1] != cv && col[v ^ 1] != cu;
         const int ur = (u >> 1) * 2, vr = (v >> 1) * 2;
@@ -782,15 +805,17 @@ static int64_t walk_slot_learned(int n, int steps, uint64_t seed, uint32_t sweep
         uint32_t s1n = 0;
         for (int i = 0; i < m; ++i) {
             nb[i] = bad[i] + db[i];
-            w2[i] = wt[ROW(score + delta) + (nb[i] < 63 ? nb[i] : 63)];
+            w2[i] = clamp_weight(wt[ROW(score + delta) + (nb[i] < 63 ? nb[i] : 63)]);
             s1n += w2[i];
         }
-        const uint64_t big_n = 2ull * s1 + 4ull * (uint64_t)m * ((uint64_t)w[u] + w[v]);
-        const uint64_t big_n2 = 2ull * s1n + 4ull * (uint64_t)m * ((uint64_t)w2[u] + w2[v]);
+        /* pair probability 2 (0.2/m^2 + 0.8 w(u) w(v)/S1^2) = 0.4 N / (m^2 S1^2);
+         * accept iff r N S1'^2 < thr64(delta) N' S1^2 */
+        const uint64_t mm4 = 4ull * (uint64_t)m * (uint64_t)m;
+        const uint64_t big_n = (uint64_t)s1 * s1 + mm4 * w[u] * w[v];
+        const uint64_t big_n2 = (uint64_t)s1n * s1n + mm4 * w2[u] * w2[v];
         const int d = delta < -k64 ? -k64 : (delta > k64 - 1 ? k64 - 1 : delta);
-        const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * s1n;
-        const unsigned __int128 rhs = (unsigned __int128)thr64[d + k64] * big_n2 * s1;
-        if (!(lhs < rhs)) continue;
+        const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * ((uint64_t)s1n * s1n);
+        if (!below_product(lhs, (unsigned __int128)thr64[d + k64] * big_n2, (uint64_t)s1 * s1)) continue;
         col[u] = cv;
         col[v] = cu;
         score += delta;

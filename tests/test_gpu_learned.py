@@ -93,16 +93,17 @@ def test_weight_table(pkg, oracle, dev):
     net = _net(pkg, dev, rng)
     log_temps = np.log(np.array([0.05, 0.3, 2.0], np.float32))
     table = net.weight_table(log_temps, max_energy=40).cpu().numpy().view(np.uint32)
-    assert table.shape == (3, 41, 64) and table.min() >= 1 and table.max() <= 1 << 21
+    assert table.shape == (3, 41, 64) and table.min() >= 1 and table.max() <= 1 << 15
     L = pkg.learned
     for t, lt in enumerate(log_temps):
         for e in (0, 7, 40):
             x = L.to_bf16(L.point_features(np.arange(64), lt, e))
-            g = np.clip(oracle.mlp_forward(net.params.cpu().numpy().view(np.uint16), x), -6, 6)
-            w = np.clip(np.rint(4096 * np.exp(g)), 1, 1 << 21)
+            g = np.clip(oracle.mlp_forward(net.params.cpu().numpy().view(np.uint16), x), -6, 5.5)
+            w = np.clip(np.rint(128 * np.exp(g)), 1, 1 << 15)
             # the device computes the features with its own sqrt/log before
-            # bf16 rounding: allow a few bf16 ulps of the logit
-            np.testing.assert_allclose(table[t, e], w, rtol=2e-2)
+            # bf16 rounding: allow a few bf16 ulps of the logit (and one unit
+            # of the integer rounding)
+            np.testing.assert_allclose(table[t, e], w, rtol=2e-2, atol=1)
 
 
 def _cols(oracle, n, replicas, seed):

@@ -171,24 +171,42 @@ def learned_ab(pkg, seconds: float) -> dict:
 NEAR_SOLUTION = ROOT / "tests" / "golden" / "best-near-solution-E10.json"
 
 
-def learned_ab_near_solution(pkg, seconds: float, moves: int = 12) -> dict | None:
-    """The same A/B from the reference's saved E=10 near-solution, each
-    replica perturbed by `moves` random legal 2-switches (seeded): a setting
-    in which E <= 12 is reachable within the budget, so time-to-E<=12 is
-    measured rather than null."""
+def learned_ab_near_solution(pkg, seconds: float, moves: int = 12, seeds=(12, 13, 14)) -> dict | None:
+    """The A/B from the reference's saved E=10 near-solution, each replica
+    perturbed by `moves` random legal 2-switches (seeded), where E <= 12 is
+    reachable within the budget so time-to-E<=12 is measured.  The network is
+    trained beforehand on the logs of a plain calibration run from another
+    perturbation (its time is reported, not counted); online retraining
+    still runs inside the learned side's budget.  One race per seed."""
     import numpy as np
 
     if not NEAR_SOLUTION.exists():
         return None
     _, points, _ = pkg.io.load_candidate(NEAR_SOLUTION, 75)
     state = pkg.points_to_permutations(points, 75)
-    base = np.stack((np.array(state.P), np.array(state.Q)), 1).reshape(1, 150).astype(np.int32)
-    start = pkg.random_two_switches(np.repeat(base, 4096, 0), moves, seed=moves)
-    res = pkg.learned.ab_run(75, 4096, 64, seconds=seconds, seed=11, state_seed=1, train_sweeps=300, chunk=100,
-                             thresholds=(40, 30, 20, 15, 12, 11, 10), start=start)
-    out = _ab_summary(res)
-    out["start"] = f"reference near-solution (E=10) + {moves} random 2-switches per replica"
-    return out
+    base = np.repeat(np.stack((np.array(state.P), np.array(state.Q)), 1).reshape(1, 150).astype(np.int32), 4096, 0)
+    net, calibration = pkg.learned.pretrain(75, 4096, 64, seed=11, start=pkg.random_two_switches(base, moves, seed=99))
+    races = []
+    for s in seeds:
+        res = pkg.learned.ab_run(75, 4096, 64, seconds=seconds, seed=11, state_seed=1, chunk=50,
+                                 thresholds=(30, 20, 15, 12, 11, 10), start=pkg.random_two_switches(base, moves, seed=s),
+                                 network=net.clone())
+        summary = _ab_summary(res)
+        summary["perturbation_seed"] = s
+        races.append(summary)
+    t_u = [r["time_to_E12"]["uniform"] for r in races]
+    t_l = [r["time_to_E12"]["learned"] for r in races]
+
+    def median(values):
+        values = sorted(v if v is not None else float("inf") for v in values)
+        mid = values[len(values) // 2]
+        return None if mid == float("inf") else mid
+
+    return {"start": f"reference near-solution (E=10) + {moves} random 2-switches per replica",
+            "budget_seconds_each": seconds, "calibration": {k: calibration[k] for k in ("seconds", "tuples", "sweeps",
+                                                                                     "steps")},
+            "races": races, "time_to_E12_median": {"uniform": median(t_u), "learned": median(t_l)},
+            "learned_first": sum(1 for a, b in zip(t_u, t_l) if b is not None and (a is None or b < a))}
 
 
 def search_rates(pkg) -> dict:
@@ -238,7 +256,7 @@ def main() -> int:
     ap.add_argument("--legs", default="temper,batched,learned,near,search",
                     help="comma list of the extra legs to run: temper, batched, learned, near, search")
     ap.add_argument("--ab-seconds", type=float, default=20.0, help="wall-clock budget of each A/B side")
-    ap.add_argument("--near-seconds", type=float, default=8.0,
+    ap.add_argument("--near-seconds", type=float, default=4.0,
                     help="wall-clock budget of each side of the near-solution A/B")
     ap.add_argument("--method", type=int, default=0, help="0 auto (direction histograms), 1 determinant")
     ap.add_argument("--calibrate-copy", action="store_true",

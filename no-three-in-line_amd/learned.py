@@ -91,6 +91,15 @@ class ProposalNetwork:
     def _stream(self) -> int:
         return _lib.stream_handle(self.device)
 
+    def clone(self) -> "ProposalNetwork":
+        """An independent copy (parameters, master weights, Adam state)."""
+        other = ProposalNetwork.__new__(ProposalNetwork)
+        other.device = self.device
+        for name in ("master", "params", "m", "v"):
+            setattr(other, name, getattr(self, name).clone())
+        other.step = self.step
+        return other
+
     def forward(self, x_bf16: torch.Tensor) -> torch.Tensor:
         """Logits of feature rows (int16 tensor holding bf16 bits, [N, 8])."""
         rows = int(x_bf16.shape[0])
@@ -266,7 +275,8 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
            train_sweeps: int = 200, log_sweeps: int = 2, train_steps: int = 400, retrain_seconds: float = 2.0,
            retrain_steps: int = 100, log_capacity: int = 1 << 21,
            chunk: int = 50, thresholds=(80, 70, 60, 55, 50, 45, 40, 12), tmin: float = 0.05, tmax: float = 2.0,
-           start: np.ndarray | None = None, device: torch.device | None = None) -> dict:
+           start: np.ndarray | None = None, network: "ProposalNetwork | None" = None,
+           device: torch.device | None = None) -> dict:
     """Uniform vs learned proposals on the same seeds, replicas and GPU.
 
     Both start from the same states: seeded generator states, or the rows of
@@ -275,8 +285,12 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
     sweeps, the last `log_sweeps` of them logged, trains the network on that
     log (`train_steps` Adam steps) and continues with learned proposals; every
     `retrain_seconds` it logs `log_sweeps` more plain sweeps and takes
-    `retrain_steps` more steps on them (online training).  The trace records (elapsed seconds, best energy) after every
-    chunk; logging and training time count against the learned run."""
+    `retrain_steps` more steps on them (online training).  With `network`
+    given (trained beforehand, e.g. by `pretrain` on another run's logs) the
+    learned run starts with learned proposals at once and only the online
+    retraining happens inside its budget.  The trace records (elapsed
+    seconds, best energy) after every chunk; logging and training time count
+    against the learned run."""
     from .batched import seeded_chains
 
     dev = _lib.device(None if device is None else device.index)
@@ -311,7 +325,7 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
                          "threshold_seconds": threshold_times(trace_u, thresholds)}
     # learned
     lrn = chains()
-    net = ProposalNetwork(seed=seed, device=dev)
+    net = ProposalNetwork(seed=seed, device=dev) if network is None else network
     log = ProposalLog(log_capacity, dev)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
@@ -325,17 +339,19 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
         lrn.log = None
         sweeps += count
 
-    done = 0
-    while done < train_sweeps - log_sweeps:
-        step = min(chunk, train_sweeps - log_sweeps - done)
-        plain(step, False)
-        done += step
-        torch.cuda.synchronize(dev)
-        trace_l.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
-    plain(log_sweeps, True)
-    xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev)
-    tuples = int(y.shape[0])
-    losses = net.train(xu, xv, y, steps=train_steps, seed=seed)
+    tuples, losses = 0, []
+    if network is None:
+        done = 0
+        while done < train_sweeps - log_sweeps:
+            step = min(chunk, train_sweeps - log_sweeps - done)
+            plain(step, False)
+            done += step
+            torch.cuda.synchronize(dev)
+            trace_l.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
+        plain(log_sweeps, True)
+        xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev)
+        tuples = int(y.shape[0])
+        losses = net.train(xu, xv, y, steps=train_steps, seed=seed)
     proposals = prepare(lrn, net)
     torch.cuda.synchronize(dev)
     train_seconds = time.perf_counter() - t0
@@ -363,5 +379,35 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
     result["config"] = {"n": n, "replicas": replicas, "ladder": ladder, "seconds": seconds, "seed": seed,
                         "state_seed": state_seed if start is None else None, "tmin": tmin, "tmax": tmax,
                         "train_sweeps": train_sweeps, "log_sweeps": log_sweeps, "train_steps": train_steps,
-                        "retrain_seconds": retrain_seconds, "retrain_steps": retrain_steps}
+                        "retrain_seconds": retrain_seconds, "retrain_steps": retrain_steps,
+                        "pretrained": network is not None}
     return result
+
+
+def pretrain(n: int, replicas: int, ladder: int, *, seed: int, start: np.ndarray | None = None, state_seed: int = 2,
+             sweeps: int = 300, log_sweeps: int = 2, steps: int = 400, tmin: float = 0.05, tmax: float = 2.0,
+             device: torch.device | None = None) -> tuple["ProposalNetwork", dict]:
+    """A network trained on the logs of a separate plain calibration run
+    (`sweeps` uniform sweeps from `start` or seeded states, the last
+    `log_sweeps` logged, `steps` Adam steps): returns it with the run's
+    timing."""
+    from .batched import seeded_chains
+
+    dev = _lib.device(None if device is None else device.index)
+    t0 = time.perf_counter()
+    if start is None:
+        chains = seeded_chains(n, replicas, ladder, seed=seed, state_seed=state_seed, tmin=tmin, tmax=tmax, device=dev)
+    else:
+        temps = np.tile(np.geomspace(tmin, tmax, ladder), replicas // ladder).astype(np.float32)
+        chains = TemperingChains(np.ascontiguousarray(start, dtype=np.int32), temps, seed, device=dev)
+    chains.run(sweeps - log_sweeps, 2 * n, ladder)
+    log = ProposalLog(log_sweeps * 2 * n * replicas, dev)
+    chains.log = log
+    chains.run(log_sweeps, 2 * n, ladder)
+    chains.log = None
+    xu, xv, y = training_tensors(log.rows(), chains.temperatures_host, dev)
+    net = ProposalNetwork(seed=seed, device=dev)
+    losses = net.train(xu, xv, y, steps=steps, seed=seed)
+    torch.cuda.synchronize(dev)
+    return net, {"seconds": time.perf_counter() - t0, "tuples": int(y.shape[0]), "losses": losses,
+                 "sweeps": sweeps, "steps": steps}

@@ -217,10 +217,13 @@ int n3_mlp_weight_table(const uint16_t* params, const float* log_temps, int temp
 int n3_metropolis_thresholds64(const float* host_temperatures, int replicas, int k, uint64_t* host_out);
 
 /* `sweeps` x (learned-proposal walk of `steps` proposals; exchange(ladder)
- * if ladder > 1), graph-captured.  Slot r's first moved point is drawn from
- * 0.2 uniform + 0.8 weights[temp_index[r]][E][badness], the second uniformly
- * from the others; acceptance is the Metropolis-Hastings test on the exact
- * integer delta, in 128-bit integer arithmetic (see csrc/learned.hip). */
+ * if ladder > 1), graph-captured.  A proposal of slot r is, with probability
+ * 0.2, a uniform pair of slots, otherwise both slots drawn in proportion to
+ * w = clamp(weights[temp_index[r]][E][badness], 1, 2^15) (the softmax of the
+ * network's additive pair logit over all pairs); acceptance is the
+ * Metropolis-Hastings test on the exact integer delta, evaluated exactly in
+ * integers (see csrc/learned.hip).  No counterpart in the reference (the
+ * nearest is bad_bias, tempering.py:51-55). */
 int n3_temper_run_learned(const n3_chains* chains, const uint32_t* weights, const int32_t* temp_index,
                           int max_energy, const uint64_t* thr64, int k64, int sweeps, int steps, int ladder,
                           uint64_t seed, void* stream);

@@ -198,7 +198,9 @@ int n3_mlp_forward(const uint16_t* params, const uint16_t* x, int64_t rows, floa
 
 /* Gradient of the mean pair loss BCE(sigmoid(g(xu) + g(xv) - b2), y) over
  * `pairs` tuples, ADDED to grad (fp32[n3_mlp_params()]); the summed loss is
- * added to loss[0]. */
+ * added to loss[0].  Per-wavefront partial sums are added in a fixed order,
+ * so the result is deterministic.  Calls that overlap on one device race on
+ * the library's partial-sum buffer. */
 int n3_mlp_grad(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y, int64_t pairs,
                 float* grad, float* loss, void* stream);
 

@@ -132,19 +132,33 @@ __global__ void __launch_bounds__(256) mlp_forward_kernel(const uint16_t* __rest
 }
 
 // Pair-logit BCE gradient: tuple i has rows xu[i], xv[i] and label y[i];
-// z = g(xu) + g(xv), loss = mean BCE(sigmoid(z), y).  Gradients of the mean
-// loss are added to grad[kParams] (fp32) and the summed loss to loss[0].
-__global__ void __launch_bounds__(64) mlp_grad_kernel(const uint16_t* __restrict__ params,
-                                                      const uint16_t* __restrict__ xu,
-                                                      const uint16_t* __restrict__ xv, const float* __restrict__ y,
-                                                      long long pairs, float* __restrict__ grad,
-                                                      float* __restrict__ loss) {
+// z = g(xu) + g(xv), loss = mean BCE(sigmoid(z), y).  Every wavefront takes
+// tiles of 32 tuples in a grid stride and keeps its gradient in registers;
+// the block adds its wavefronts' gradients (and summed losses) in LDS in
+// wavefront order and stores them as column `block` of `partial`
+// ([kParams + 1][kGradMaxBlocks]); grad_reduce_kernel then adds each
+// column's blocks in block order, so the gradient is deterministic (no
+// floating-point atomics).
+constexpr int kGradWaves = 4;         // wavefronts per block
+constexpr int kGradMaxBlocks = 128;   // columns of the partial-gradient buffer
+constexpr int kPartial = kParams + 1; // gradient, then the summed loss
+
+__global__ void __launch_bounds__(256) mlp_grad_kernel(const uint16_t* __restrict__ params,
+                                                       const uint16_t* __restrict__ xu,
+                                                       const uint16_t* __restrict__ xv, const float* __restrict__ y,
+                                                       long long pairs, float* __restrict__ partial) {
   __shared__ float b1[kHidden], w2[kHidden];
-  __shared__ uint16_t dh_t[32 * 32];  // one tile of dH, [hidden 32][rows 32]
+  __shared__ uint16_t dh_tile[kGradWaves][32 * 32];  // one tile of dH, [hidden 32][rows 32]
+  __shared__ uint16_t x_tile[kGradWaves][32 * kIn];  // the tile's features, [row][feature]
+  __shared__ float wave_grad[kGradWaves][kPartial];   // the wavefronts' gradients
   stage_layer_params(params, b1, w2);
   __syncthreads();
   const float b2 = bf16_to_float(params[kB2]);
-  const int lane = lane_id(), half = lane >> 5, j = lane & 31;
+  const int wave = threadIdx.x / kWave, lane = lane_id(), half = lane >> 5, j = lane & 31;
+  uint16_t* dh_t = dh_tile[wave];
+  uint16_t* xt = x_tile[wave];
+  const long long my_wave = (long long)blockIdx.x * kGradWaves + wave;
+  const long long all_waves = (long long)gridDim.x * kGradWaves;
   const float inv_pairs = 1.0f / (float)pairs;
   floatx16_t dw1[4];  // dW1 tiles: [hidden 32][feature 32 (8 used)]
   float dw2[4][16], db1[4][16];
@@ -158,7 +172,7 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(const uint16_t* __restrict
     }
   float db2 = 0.0f, loss_sum = 0.0f;
   const long long tiles = (pairs + 31) / 32;
-  for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  for (long long tile = my_wave; tile < tiles; tile += all_waves) {
     const long long pair = tile * 32 + j;
     const bool live = pair < pairs;
     const bf16x4_t x_u = load_row_features(xu, pair, pairs);
@@ -180,6 +194,17 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(const uint16_t* __restrict
     // both towers: dh = dz * w2 * [pre > 0]; accumulate dw2, db1, and dW1
     for (int side = 0; side < 2; ++side) {
       const bf16x4_t xs = side == 0 ? x_u : x_v;
+      // the MFMA B operand (rows x features) of the four K=8 steps, from
+      // the tile's features staged in LDS: lane l supplies rows
+      // 8 s + 4 half .. +3 of feature l % 32 (zero past the 8 features)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xt[j * kIn + 4 * half + i] = (uint16_t)xs[i];
+      wave_sync();
+      bf16x4_t bv[4];
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bv[st][i] = j < kIn ? (short)xt[(8 * st + 4 * half + i) * kIn + j] : (short)0;
       floatx16_t a[4];
       hidden_tile(params, xs, a);  // recomputed: cheaper than keeping both sides live
 #pragma unroll
@@ -192,64 +217,44 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(const uint16_t* __restrict
           const float dh = pre > 0.0f ? dz * w2[h] : 0.0f;
           dw2[t][r] += dz * act;
           db1[t][r] += dh;
-          // dH tile in LDS as [hidden (within tile)][row]
           dh_t[(8 * (r >> 2) + 4 * half + (r & 3)) * 32 + j] = float_to_bf16(dh);
         }
-        __syncthreads();
+        wave_sync();
         // dW1 tile t += dH_t (32 hidden x 32 rows) X (32 rows x 8 features):
-        // four K=8 steps over the rows; lane l supplies A[hidden l%32][rows
-        // 8s + 4 half ..+3] and B[rows 8s + 4 half ..+3][feature l%32]
+        // lane l supplies A[hidden l % 32][rows 8 s + 4 half .. +3]
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          bf16x4_t av4, bv4;
+        for (int st = 0; st < 4; ++st) {
+          bf16x4_t av4;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int rr = 8 * s + 4 * half + i;
-            av4[i] = (short)dh_t[j * 32 + rr];
-            // feature j of row rr: exchanged through the lanes that hold row rr
-            const int src = rr;  // lanes 0..31 hold rows 0..31 (both halves hold the row's features)
-            short fv = 0;
-            {
-              // features 0..3 live in lane rr (half 0), 4..7 in lane rr + 32
-              const int k = j;
-              const int owner = (k < 4) ? src : src + 32;
-              const int comp = k & 3;
-              short c0 = (short)__shfl((int)xs[0], owner, kWave);
-              short c1 = (short)__shfl((int)xs[1], owner, kWave);
-              short c2 = (short)__shfl((int)xs[2], owner, kWave);
-              short c3 = (short)__shfl((int)xs[3], owner, kWave);
-              const short pick = comp == 0 ? c0 : (comp == 1 ? c1 : (comp == 2 ? c2 : c3));
-              fv = k < kIn ? pick : (short)0;
-            }
-            bv4[i] = fv;
-          }
-          dw1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av4, bv4, dw1[t], 0, 0, 0);
+          for (int i = 0; i < 4; ++i) av4[i] = (short)dh_t[j * 32 + 8 * st + 4 * half + i];
+          dw1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av4, bv[st], dw1[t], 0, 0, 0);
         }
-        __syncthreads();
+        wave_sync();
       }
     }
   }
-  // dW1: lane l holds dW1[hidden 32t + row(r, half)][feature l%32]
+  float* out = wave_grad[wave];
+  // dW1: lane l holds dW1[hidden 32t + row(r, half)][feature l % 32]
   if (j < kIn) {
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) atomicAdd(&grad[kW1 + hidden_of(t, r, half) * kIn + j], dw1[t][r]);
+      for (int r = 0; r < 16; ++r) out[kW1 + hidden_of(t, r, half) * kIn + j] = dw1[t][r];
   }
   // dw2, db1: sum over the 32 rows (lanes) of each half
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      float a = dw2[t][r], b = db1[t][r];
+      float sa = dw2[t][r], sb = db1[t][r];
 #pragma unroll
       for (int off = 16; off >= 1; off >>= 1) {
-        a += __shfl_xor(a, off, kWave);
-        b += __shfl_xor(b, off, kWave);
+        sa += __shfl_xor(sa, off, kWave);
+        sb += __shfl_xor(sb, off, kWave);
       }
       if (j == 0) {
-        atomicAdd(&grad[kW2 + hidden_of(t, r, half)], a);
-        atomicAdd(&grad[kB1 + hidden_of(t, r, half)], b);
+        out[kW2 + hidden_of(t, r, half)] = sa;
+        out[kB1 + hidden_of(t, r, half)] = sb;
       }
     }
 #pragma unroll
@@ -258,9 +263,64 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(const uint16_t* __restrict
     loss_sum += __shfl_xor(loss_sum, off, kWave);
   }
   if (lane == 0) {
-    atomicAdd(&grad[kB2], db2);
-    atomicAdd(loss, loss_sum);
+    out[kB2] = db2;
+    out[kParams] = loss_sum;
   }
+  __syncthreads();
+  for (int c = threadIdx.x; c < kPartial; c += blockDim.x) {
+    float sum = wave_grad[0][c];
+#pragma unroll
+    for (int w = 1; w < kGradWaves; ++w) sum += wave_grad[w][c];
+    partial[(long long)c * kGradMaxBlocks + blockIdx.x] = sum;
+  }
+}
+
+// Sum over the 64 lanes by the DPP pattern of wave_sum_dpp (fixed order).
+__device__ inline float wave_sum_f32(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xA, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xC, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+// One wavefront per parameter (and one for the loss): grad[c] += the sum of
+// column c's `blocks` entries, lane l adding entries l, l + 64, ... in order.
+__global__ void __launch_bounds__(256) grad_reduce_kernel(const float* __restrict__ partial, int blocks,
+                                                          float* __restrict__ grad, float* __restrict__ loss) {
+  const int c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (c >= kPartial) return;
+  const int lane = lane_id();
+  float part = 0.0f;
+  for (int b = lane; b < blocks; b += kWave) part += partial[(long long)c * kGradMaxBlocks + b];
+  const float sum = wave_sum_f32(part);
+  if (lane == 0) {
+    if (c < kParams)
+      grad[c] += sum;
+    else
+      loss[0] += sum;
+  }
+}
+
+// Per-device partial-gradient buffer ([kPartial][kGradMaxBlocks] floats).
+static float* grad_partials(int* st_out) {
+  static float* buffers[64] = {nullptr};
+  int device = 0;
+  (void)hipGetDevice(&device);
+  if (device < 0 || device >= 64) {
+    *st_out = set_error(N3_ENODEV, "device index %d out of range", device);
+    return nullptr;
+  }
+  if (!buffers[device]) {
+    float* b = nullptr;
+    *st_out = hip_status(hipMalloc(&b, sizeof(float) * kGradMaxBlocks * kPartial), "hipMalloc(partials)");
+    if (*st_out) return nullptr;
+    buffers[device] = b;
+  }
+  *st_out = N3_OK;
+  return buffers[device];
 }
 
 __global__ void adam_kernel(float* __restrict__ master, uint16_t* __restrict__ params, const float* __restrict__ grad,
@@ -344,11 +404,21 @@ extern "C" int n3_mlp_forward(const uint16_t* params, const uint16_t* x, int64_t
 extern "C" int n3_mlp_grad(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y,
                            int64_t pairs, float* grad, float* loss, void* stream) {
   if (!params || !xu || !xv || !y || !grad || !loss || pairs < 1) return set_error(N3_EINVAL, "bad grad arguments");
-  long long blocks = (pairs + 31) / 32;
-  if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3((unsigned)blocks), dim3(64), 0, as_stream(stream), params, xu, xv, y,
-                     (long long)pairs, grad, loss);
-  return hip_status(hipGetLastError(), "mlp_grad_kernel launch");
+  int st = N3_OK;
+  float* partial = grad_partials(&st);
+  if (st) return st;
+  const long long tiles = (pairs + 31) / 32;
+  long long blocks = (tiles + kGradWaves - 1) / kGradWaves;
+  if (blocks > kGradMaxBlocks) blocks = kGradMaxBlocks;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3((unsigned)blocks), dim3(kGradWaves * kWave), 0, s, params, xu, xv, y,
+                     (long long)pairs, partial);
+  st = hip_status(hipGetLastError(), "mlp_grad_kernel launch");
+  if (st) return st;
+  const int reduce_waves = 256 / kWave;
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3((kPartial + reduce_waves - 1) / reduce_waves), dim3(256), 0, s, partial,
+                     (int)blocks, grad, loss);
+  return hip_status(hipGetLastError(), "grad_reduce_kernel launch");
 }
 
 extern "C" int n3_mlp_adam(float* master, uint16_t* params, const float* grad, float* m, float* v, int step, float lr,

@@ -88,6 +88,36 @@ def test_gradient_matches_float64(pkg, oracle, dev):
         assert abs(loss - ref_loss / pairs) < 1e-4 * max(1.0, abs(ref_loss / pairs))
 
 
+def test_gradient_is_deterministic_and_additive(pkg, oracle, dev):
+    rng = np.random.default_rng(9)
+    net = _net(pkg, dev, rng)
+    pairs = 70000  # more tiles than the partial-sum rows: grid-stride tiles
+    xu, xv = _features(pkg, rng, pairs), _features(pkg, rng, pairs)
+    y = rng.random(pairs).astype(np.float32)
+    args = [torch.from_numpy(xu.view(np.int16)).to(dev), torch.from_numpy(xv.view(np.int16)).to(dev),
+            torch.from_numpy(y).to(dev)]
+    g1, l1 = net.gradient(*args)
+    g2, l2 = net.gradient(*args)
+    assert torch.equal(g1, g2) and l1 == l2
+    acc = torch.zeros_like(g1)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    net.gradient(*args, loss=loss)
+    _lib = pkg._lib
+    _lib.check(_lib.lib().n3_mlp_grad(net.params.data_ptr(), args[0].data_ptr(), args[1].data_ptr(),
+                                      args[2].data_ptr(), pairs, acc.data_ptr(), loss.data_ptr(),
+                                      _lib.stream_handle(dev)))
+    _lib.check(_lib.lib().n3_mlp_grad(net.params.data_ptr(), args[0].data_ptr(), args[1].data_ptr(),
+                                      args[2].data_ptr(), pairs, acc.data_ptr(), loss.data_ptr(),
+                                      _lib.stream_handle(dev)))
+    torch.cuda.synchronize()
+    assert torch.allclose(acc, 2 * g1, rtol=1e-6, atol=1e-9)
+    ref, _ = oracle.mlp_pair_grad(net.params.cpu().numpy().view(np.uint16), xu, xv, y)
+    got = g1.cpu().numpy()
+    for lo, hi in ((0, 1024), (1024, 1152), (1152, 1280), (1280, 1281)):
+        scale = np.abs(ref[lo:hi]).max() + 1e-12
+        assert np.abs(got[lo:hi] - ref[lo:hi]).max() <= 2e-2 * scale, lo
+
+
 def test_weight_table(pkg, oracle, dev):
     rng = np.random.default_rng(4)
     net = _net(pkg, dev, rng)

@@ -64,3 +64,14 @@ def violating_lines(points: list[Point]) -> list[ViolatingLine]:
            for key, idx in members.items() if len(idx) >= 3]
     out.sort(key=lambda line: (-line.triples, line.key))
     return out
+
+
+def primitive_slope(key: LineKey) -> tuple[int, int]:
+    """geometry.py:33-40: the line's direction (dx, dy) = (B, -A) reduced,
+    signed so that dx > 0 or dx = 0 < dy."""
+    dx, dy = key[1], -key[0]
+    g = gcd(abs(dx), abs(dy))
+    dx, dy = dx // g, dy // g
+    if dx < 0 or (dx == 0 and dy < 0):
+        dx, dy = -dx, -dy
+    return dx, dy

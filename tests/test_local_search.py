@@ -83,3 +83,34 @@ def test_breakout_with_oracle_evaluators(pkg, oracle):
     result.state.validate()
     assert result.score == oracle.score_and_badness(result.state.P, result.state.Q)[0]
     assert result.breakouts > 0 and result.weighted_lines <= 128
+
+
+def test_conflict_metrics(pkg, oracle):
+    """conflicts.analyze_points against the oracle's score and badness, a
+    hand-made configuration and the determinant recount."""
+    c = pkg.conflicts
+    rng = Random(11)
+    for n in (6, 12, 30):
+        state = pkg.random_state(n, rng)
+        points = [(r, state.P[r]) for r in range(n)] + [(r, state.Q[r]) for r in range(n)]
+        metrics = c.analyze_points(points)
+        score, bad = oracle.score_and_badness(np.array(state.P), np.array(state.Q))
+        assert metrics.triples == score == c.full_triple_score(points)
+        # badness is reported in the input order: P points then Q points
+        slot_order = [2 * r for r in range(n)] + [2 * r + 1 for r in range(n)]
+        assert list(metrics.badness) == [int(bad[s]) for s in slot_order]
+        assert sum(metrics.slope_histogram.values()) == score
+        assert metrics.lexicographic[0] == score
+    # four points on the diagonal and one off it: one line, C(4,3) triples
+    pts = [(0, 0), (1, 1), (2, 2), (3, 3), (0, 3)]
+    m = c.analyze_points(pts, {pkg.lines.line_key((0, 0), (1, 1)): 5})
+    assert (m.triples, m.violating_lines, m.max_line_occupancy, m.max_point_badness) == (4, 1, 4, 3)
+    assert m.weighted_score == 20 and m.badness == (3, 3, 3, 3, 0) and m.slope_histogram == {(1, 1): 4}
+    assert c.full_triple_score([(0, 0), (0, 0), (1, 1)]) == 1  # duplicates: zero determinant
+    state = pkg.random_state(10, Random(3))
+    for layer in (0, 1):
+        delta = c.reference_swap_delta(state, layer, 2, 7)
+        moved = state.swapped(layer, 2, 7)
+        after = oracle.score_and_badness(np.array(moved.P), np.array(moved.Q))[0]
+        before = oracle.score_and_badness(np.array(state.P), np.array(state.Q))[0]
+        assert delta == after - before

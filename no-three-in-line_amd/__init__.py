@@ -27,7 +27,7 @@ from .moves import apply_three_cycle, legal_three_cycle_candidates
 from .state import PairPermutations, points_to_permutations, random_state, random_two_switches
 from .tempering import TemperingChains
 from .batched import BatchGroup, run_batched, seeded_chains
-from . import autotune, breakout, conflicts, gpu_device_search, gpu_exhaustive_beam, io, learned, lines, search
+from . import autotune, breakout, conflicts, gpu_device_search, gpu_exhaustive_beam, io, learned, lines, search, verifiers
 from .breakout import BreakoutResult, breakout_search
 from .search import SearchResult, SearchSnapshot, local_search
 from .gpu_device_search import (GpuSearchResult, gpu_device_beam_search, mutate_population_device, rank_elites,
@@ -43,5 +43,5 @@ __all__ = [
     "BatchGroup", "run_batched", "seeded_chains", "autotune", "io", "learned",
     "gpu_device_search", "gpu_exhaustive_beam", "GpuSearchResult", "gpu_device_beam_search", "mutate_population_device",
     "rank_elites", "rank_elites_device", "candidate_descriptors", "gpu_exhaustive_beam_search",
-    "breakout", "conflicts", "lines", "search", "BreakoutResult", "breakout_search", "SearchResult", "SearchSnapshot", "local_search",
+    "breakout", "conflicts", "lines", "search", "verifiers", "BreakoutResult", "breakout_search", "SearchResult", "SearchSnapshot", "local_search",
 ]

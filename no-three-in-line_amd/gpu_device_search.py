@@ -147,8 +147,15 @@ def mutate_population_device(parent_p: np.ndarray, parent_q: np.ndarray, *, popu
 
 def verify_zero(state: PairPermutations) -> None:
     """The reference verifies an E=0 find with two independent verifiers
-    (gpu_device_search.py:413-418); here the determinant kernel and the
-    direction-histogram kernel must both count zero triples."""
+    (gpu_device_search.py:413-418); here both host verifiers must accept it,
+    and the determinant kernel and the direction-histogram kernel must both
+    count zero triples."""
+    from .verifiers import verify_a, verify_b
+
+    ok_a, detail_a = verify_a(state.points(), state.n)
+    ok_b, detail_b = verify_b(state.points(), state.n)
+    if not (ok_a and ok_b):
+        raise AssertionError(f"E=0 failed verification: A={detail_a}; B={detail_b}")
     dev = _lib.device()
     p = torch.tensor([state.P], dtype=torch.int32, device=dev)
     q = torch.tensor([state.Q], dtype=torch.int32, device=dev)

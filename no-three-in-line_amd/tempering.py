@@ -339,6 +339,9 @@ def main(argv=None) -> int:
             print(f"epoch={epoch} best={best} proposals={proposals} seconds={time.perf_counter()-start:.2f}",
                   flush=True)
             if best == 0:
+                from .verifiers import verify_a, verify_b
+
+                assert verify_a(state.points(), n)[0] and verify_b(state.points(), n)[0]
                 break
         if epoch % 10 == 0 or epoch == args.rounds:
             chains.validate([epoch % args.replicas])

@@ -247,6 +247,15 @@ def prepare(chains: TemperingChains, network: ProposalNetwork) -> LearnedProposa
                             torch.from_numpy(thr.view(np.int64)).to(chains.device), chains.k_move)
 
 
+def refresh(proposals: LearnedProposals, chains: TemperingChains, network: ProposalNetwork) -> LearnedProposals:
+    """Rewrites the weight table in place from the network's current
+    parameters (the tensors keep their addresses, so captured learned-walk
+    graphs stay valid and are reused)."""
+    log_temps, _ = temperature_index(chains.temperatures_host)
+    proposals.weights.copy_(network.weight_table(log_temps, proposals.max_energy))
+    return proposals
+
+
 def run_learned(chains: TemperingChains, proposals: LearnedProposals, sweeps: int, steps: int,
                 ladder_size: int = 0) -> None:
     """`sweeps` x (learned-proposal walk; exchange), graph-captured, enqueued."""
@@ -364,7 +373,7 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
             xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev)
             tuples += int(y.shape[0])
             losses += net.train(xu, xv, y, steps=retrain_steps, seed=seed + retrains + 1)
-            proposals = prepare(lrn, net)
+            proposals = refresh(proposals, lrn, net)
             torch.cuda.synchronize(dev)
             last_train = time.perf_counter() - t0
             retrains += 1

@@ -204,3 +204,13 @@ def test_training_on_logged_proposals(pkg, dev):
     pkg.learned.run_learned(chains, props, 20, 80, 64)
     torch.cuda.synchronize()
     chains.validate()
+    # online retraining rewrites the table in place (the captured graph is reused)
+    before = props.weights.clone()
+    net.train(xu, xv, y, steps=50, seed=3)
+    again = pkg.learned.refresh(props, chains, net)
+    assert again.weights.data_ptr() == props.weights.data_ptr() and not torch.equal(before, props.weights)
+    fresh = pkg.learned.prepare(chains, net)
+    assert torch.equal(fresh.weights, props.weights)
+    pkg.learned.run_learned(chains, props, 20, 80, 64)
+    torch.cuda.synchronize()
+    chains.validate()

@@ -119,3 +119,22 @@ def test_large_n_determinant_path(pkg, oracle, dev):
     scores, bad = _score(pkg, hp, hq, dev, 0)
     ref_scores, ref_bad = oracle.score_population(hp, hq)
     assert np.array_equal(scores, ref_scores) and np.array_equal(bad, ref_bad)
+
+
+def test_reflections_leave_scores_invariant_full_population(pkg, dev):
+    """n=75, P=65,536: mirroring columns (c -> n-1-c) keeps every slot's
+    badness; flipping rows (r -> n-1-r) moves each slot's badness with its
+    row; both keep every score.  A wrong sign or canonicalisation in the
+    direction table breaks these symmetries."""
+    n, count = 75, 65536
+    p, q = pkg.generate_states(0, n, count, device=dev)
+    bad = torch.empty((count, 2 * n), dtype=torch.int32, device=dev)
+    scores = pkg.score_population_device(p, q, badness=bad)
+    mp, mq = (n - 1 - p).contiguous(), (n - 1 - q).contiguous()
+    mbad = torch.empty_like(bad)
+    assert torch.equal(pkg.score_population_device(mp, mq, badness=mbad), scores)
+    assert torch.equal(mbad, bad)
+    fp, fq = p.flip(1).contiguous(), q.flip(1).contiguous()
+    fbad = torch.empty_like(bad)
+    assert torch.equal(pkg.score_population_device(fp, fq, badness=fbad), scores)
+    assert torch.equal(fbad.view(count, n, 2).flip(1).reshape(count, 2 * n), bad)

@@ -321,6 +321,230 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
   }
 }
 
+// ---------------------------------------------------------------------------
+// forward direction-histogram path (one wavefront per state)
+// ---------------------------------------------------------------------------
+
+// The forward path looks only at offsets with dr >= 0: it stages the table
+// rows dr = 0 .. n-1, from the 8-byte boundary at or below row 0's start.
+__host__ __device__ inline int forward_table_first(const DirTableInfo& info) {
+  return ((info.n - 1) * info.width) & ~3;
+}
+__host__ __device__ inline int forward_table_bytes(const DirTableInfo& info) {
+  return ((info.entries - forward_table_first(info)) * 2 + 15) / 16 * 16;
+}
+
+// Anchor a sees only the slots b > a, which lie in rows >= row(a).  After the
+// histogram of their directions is built, a slot b reads back cnt_b, the
+// number of slots > a on the line (a, b): b lies in cnt_b - 1 collinear
+// triples whose first slot is a, and a lies in sum_d C(cnt_d, 2) =
+// (sum_b (cnt_b - 1)) / 2 of them.  Every triple is counted once, at its
+// first slot, so the badness of a slot is its credits from earlier anchors
+// plus its own first-slot count (accelerated.py:34-36), with half the pairs
+// of the two-sided histogram.
+template <int CH>
+__global__ void __launch_bounds__(1024) score_forward_kernel(const int32_t* __restrict__ p,
+                                                            const int32_t* __restrict__ q, int n, long long count,
+                                                            int32_t* __restrict__ scores,
+                                                            int32_t* __restrict__ badness,
+                                                            const uint16_t* __restrict__ table_g, DirTableInfo info,
+                                                            int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int waves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  const int first = forward_table_first(info);
+  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + forward_table_bytes(info)) + wave * info.hist_words;
+  stage_table(table_g + first, table, info.entries - first);
+  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  __syncthreads();
+
+  const int m = 2 * n;
+  const int width = info.width;
+  const int offset = info.offset - first;  // staged index of (dr, dc) = key_b - key_a + offset
+  for (long long s = (long long)blockIdx.x * waves + wave; s < count; s += (long long)gridDim.x * waves) {
+    int key[CH];
+    int outside = 0, coincide = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      key[c] = 0;
+      if (k < m) {
+        const int col = (k & 1) ? q[s * n + (k >> 1)] : p[s * n + (k >> 1)];
+        outside |= (col < 0 || col >= n);
+        key[c] = (k >> 1) * width + col;
+      }
+      // slots 2r and 2r+1 sit in neighbouring lanes of the same chunk
+      const int partner = __shfl_xor(key[c], 1, kWave);
+      coincide |= (k < m) && (partner == key[c]);
+    }
+    if (__any(outside)) {
+      if (lane == 0) {
+        scores[s] = -1;
+        atomicOr(status, 1);
+      }
+      continue;
+    }
+    if (__any(coincide)) {
+      // coincident points: exact determinant count, with the histogram area
+      // as scratch for the state's columns (it is cleared again below)
+      int32_t* cols = reinterpret_cast<int32_t*>(hist);
+      int32_t* credit = cols + m;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        if (k < m) cols[k] = key[c] - (k >> 1) * width;
+      }
+      wave_sync();
+      int bad[CH];
+      determinant_state<CH>(cols, credit, m, bad);
+      int sum = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        if (k < m) {
+          sum += bad[c];
+          if (badness) badness[s * m + k] = bad[c];
+        }
+      }
+      sum = wave_sum(sum);
+      if (lane == 0) scores[s] = sum / 3;
+      for (int i = lane; i < 2 * m; i += kWave) hist[i] = 0;
+      wave_sync();
+      continue;
+    }
+
+    // key2: byte offset of a key in the staged table (2-byte entries)
+    int key2[CH], counts[CH], own[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      key2[c] = 2 * key[c];
+      counts[c] = own[c] = 0;
+    }
+    const bool tail_on = lane + kWave * (CH - 1) < m;
+    const unsigned char* table_bytes = reinterpret_cast<const unsigned char*>(table);
+    // Lanes without a forward slot look up the zero vector's entry (one
+    // broadcast address), read back and clear its word and skip only the
+    // increment: the zero vector's byte stays 0, so their read-back adds 0
+    // and no LDS step waits behind a branch.
+    const unsigned char* idle_entry = table_bytes + 2 * offset;
+    auto on = [&](int ca, int c, int l) {
+      bool v = (c == ca) ? lane > l : true;
+      if (c == CH - 1) v = v && tail_on;
+      return v;
+    };
+    // One anchor (slot 64 ca + l) is: lookup of the forward slots'
+    // directions, histogram increments, read-back of each slot's count and
+    // clear.  Chunks before ca hold no forward slot; in chunk ca only the
+    // lanes past l do.  The next anchor's lookups (table only) issue together
+    // with this anchor's read-backs and clears, so each anchor waits on LDS
+    // once.  The shift (id << 3) is used unmasked: the shifts and bit-field
+    // extracts read its low five bits.
+    auto lookup = [&](int ca, int l, uint32_t(&id)[CH]) {
+      const unsigned char* from = table_bytes + (2 * offset - __builtin_amdgcn_readlane(key2[ca], l));
+#pragma unroll
+      for (int c = ca; c < CH; ++c) {
+        const unsigned char* at = on(ca, c, l) ? from + key2[c] : idle_entry;
+        id[c] = *reinterpret_cast<const uint16_t*>(at);
+      }
+    };
+    auto increment = [&](int ca, int l, const uint32_t(&id)[CH]) {
+#pragma unroll
+      for (int c = ca; c < CH; ++c)
+        if (on(ca, c, l)) atomicAdd(&hist[id[c] >> 2], 1u << (id[c] << 3));
+      wave_sync();
+    };
+    auto collect = [&](int ca, const uint32_t(&id)[CH], uint32_t(&word)[CH]) {
+#pragma unroll
+      for (int c = ca; c < CH; ++c) word[c] = hist[id[c] >> 2];
+#pragma unroll
+      for (int c = ca; c < CH; ++c) hist[id[c] >> 2] = 0u;
+    };
+    // sum of the lanes' counts; each slot keeps its own running sum
+    auto accumulate = [&](int ca, const uint32_t(&id)[CH], const uint32_t(&word)[CH]) {
+      uint32_t part = 0;
+#pragma unroll
+      for (int c = ca; c < CH; ++c) {
+        const uint32_t cnt = __builtin_amdgcn_ubfe(word[c], id[c] << 3, 8);
+        counts[c] += (int)cnt;
+        part += cnt;
+      }
+      return part;
+    };
+#pragma unroll
+    for (int ca = 0; ca < CH; ++ca) {
+      // the last slot has no forward slots
+      const int limit = min(kWave, m - 1 - kWave * ca);
+      uint32_t id0[CH], id1[CH], word0[CH], word1[CH];
+      if (limit > 0) lookup(ca, 0, id0);
+      wave_sync();
+      for (int l = 0; l < limit; l += 2) {
+        increment(ca, l, id0);
+        collect(ca, id0, word0);
+        if (l + 1 < limit) lookup(ca, l + 1, id1);
+        wave_sync();
+        const uint32_t p0 = accumulate(ca, id0, word0);
+        uint32_t p1 = 0;
+        if (l + 1 < limit) {
+          increment(ca, l + 1, id1);
+          collect(ca, id1, word1);
+          if (l + 2 < limit) lookup(ca, l + 2, id0);
+          wave_sync();
+          p1 = accumulate(ca, id1, word1);
+        }
+        // two anchors per reduction: an anchor's sum of counts is at most
+        // (m - 1)^2 < 2^16, so they share a word.  Anchor a has m - 1 - a
+        // forward slots; its first-slot triples are (sum - (m - 1 - a)) / 2.
+        const uint32_t total = wave_sum_dpp(p0 | (p1 << 16));
+        const int a = kWave * ca + l;
+        const int first_own = (int)((total & 0xFFFFu) - (uint32_t)(m - 1 - a)) >> 1;
+        const int second_own = (int)((total >> 16) - (uint32_t)(m - 2 - a)) >> 1;
+        own[ca] = (lane == l) ? first_own : own[ca];
+        own[ca] = (lane == l + 1) ? second_own : own[ca];
+      }
+    }
+    // slot k was a forward slot of the anchors 0 .. k-1, and each of them
+    // adds its count, which includes k itself
+    int sum = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int k = lane + kWave * c;
+      if (k < m) {
+        const int b = counts[c] - k + own[c];
+        sum += b;
+        if (badness) badness[s * m + k] = b;
+      }
+    }
+    sum = wave_sum(sum);
+    if (lane == 0) scores[s] = sum / 3;
+  }
+}
+
+template <int CH>
+static int launch_forward(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
+                          const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
+  hipDeviceProp_t prop;
+  int device = 0;
+  (void)hipGetDevice(&device);
+  int st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (st) return st;
+  int lds_limit = (int)prop.sharedMemPerBlock;
+  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
+  const int table_bytes = forward_table_bytes(info);
+  const int hist_bytes = info.hist_words * 4;
+  int waves = (lds_limit - table_bytes) / hist_bytes;
+  if (waves > 16) waves = 16;
+  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
+  const size_t lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
+  long long blocks = ((long long)count + waves - 1) / waves;
+  const long long max_blocks = (long long)prop.multiProcessorCount;
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(score_forward_kernel<CH>, dim3((unsigned)blocks), dim3(waves * kWave), lds, stream, p, q, n,
+                     (long long)count, scores, badness, table, info, status);
+  return hip_status(hipGetLastError(), "score_forward_kernel launch");
+}
+
 template <int CH>
 static int launch_slope(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
                         const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
@@ -408,10 +632,10 @@ int score_population_impl(const int32_t* p, const int32_t* q, int n, int64_t cou
     st = device_dirtable(n, &table, &info);
     if (st) return st;
     switch (ch) {
-      case 1: st = launch_slope<1>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      case 2: st = launch_slope<2>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      case 3: st = launch_slope<3>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      default: st = launch_slope<4>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 1: st = launch_forward<1>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 2: st = launch_forward<2>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 3: st = launch_forward<3>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      default: st = launch_forward<4>(p, q, n, count, scores, badness, table, info, status, stream); break;
     }
   } else {
     return set_error(N3_EINVAL, "unknown scoring method %d", method);

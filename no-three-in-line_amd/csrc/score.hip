@@ -15,6 +15,10 @@
 // States with coincident points (P[r] == Q[r]) have no direction for the
 // pair; those states take the determinant path, which counts them exactly
 // like the reference (gpu.py:19-33).
+#include <mutex>
+#include <type_traits>
+#include <vector>
+
 #include "runtime.h"
 #include "wave_ops.h"
 
@@ -322,46 +326,88 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
 }
 
 // ---------------------------------------------------------------------------
-// forward direction-histogram path (one wavefront per state)
+// one-sided direction-histogram path (one wavefront per state)
 // ---------------------------------------------------------------------------
 
-// The forward path looks only at offsets with dr >= 0: it stages the table
-// rows dr = 0 .. n-1, from the 8-byte boundary at or below row 0's start.
-__host__ __device__ inline int forward_table_first(const DirTableInfo& info) {
-  return ((info.n - 1) * info.width) & ~3;
-}
-__host__ __device__ inline int forward_table_bytes(const DirTableInfo& info) {
-  return ((info.entries - forward_table_first(info)) * 2 + 15) / 16 * 16;
-}
+// Anchor a sees only the slots b < a, all in rows <= row(a), so the path
+// stages only the direction-table rows dr = -(n-1) .. 0.
+// LDS layouts of the one-sided path.
+//  kHalf16:   the 2-byte direction ids of the half table (row-major, W per
+//             row), then one byte-counter histogram per wave.
+//  kPacked16: the same table in a static 160 KiB array at byte 0 with wave
+//             w's histogram at byte (w + 4) * 8192, so that a counter word's
+//             address is the OR of the wave's base and the word's byte offset.
+//  kWide32:   4-byte entries (counter word's byte offset << 16 | bit shift of
+//             the counter in the word), laid out with dr fastest and a column
+//             stride S = 16 (mod 32) words: the points of 16 consecutive rows
+//             that a half-wave looks up fall in distinct bank pairs, where
+//             the row-major ids fall in random banks.
+enum OneSidedLayout { kHalf16 = 0, kPacked16 = 1, kWide32 = 2 };
+constexpr int kPackedTableBytes = 32 * 1024;
+constexpr int kPackedHistBytes = 8 * 1024;
+constexpr int kPackedWaves = 16;
 
-// Anchor a sees only the slots b > a, which lie in rows >= row(a).  After the
-// histogram of their directions is built, a slot b reads back cnt_b, the
-// number of slots > a on the line (a, b): b lies in cnt_b - 1 collinear
-// triples whose first slot is a, and a lies in sum_d C(cnt_d, 2) =
-// (sum_b (cnt_b - 1)) / 2 of them.  Every triple is counted once, at its
-// first slot, so the badness of a slot is its credits from earlier anchors
-// plus its own first-slot count (accelerated.py:34-36), with half the pairs
-// of the two-sided histogram.
-template <int CH>
-__global__ void __launch_bounds__(1024) score_forward_kernel(const int32_t* __restrict__ p,
-                                                            const int32_t* __restrict__ q, int n, long long count,
-                                                            int32_t* __restrict__ scores,
-                                                            int32_t* __restrict__ badness,
-                                                            const uint16_t* __restrict__ table_g, DirTableInfo info,
-                                                            int* __restrict__ status) {
-  extern __shared__ __align__(16) unsigned char smem[];
+struct OneSidedTable {
+  const void* data;   // entries of rows dr = -(n-1) .. 0
+  int bytes;          // staged bytes (multiple of 16)
+  int row_stride;     // entries between rows
+  int col_stride;     // entries between columns
+};
+
+// After anchor a's histogram of the directions to the slots b < a is built,
+// each b reads back cnt_b, the number of slots < a on the line (a, b): b
+// lies in cnt_b - 1 collinear triples whose last slot is a, and a lies in
+// sum_d C(cnt_d, 2) = (sum_b (cnt_b - 1)) / 2 of them.  Every triple is
+// counted once, at its last slot, so the badness of a slot is its credits
+// from later anchors plus its own last-slot count (accelerated.py:34-36),
+// with half the pairs of the two-sided histogram.  Anchors in the last,
+// partly filled chunk are the only ones that touch it.
+template <int CH, int kLayout>
+__global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __restrict__ p,
+                                                             const int32_t* __restrict__ q, int n, long long count,
+                                                             int32_t* __restrict__ scores,
+                                                             int32_t* __restrict__ badness, OneSidedTable tab,
+                                                             DirTableInfo info, int* __restrict__ status) {
+  using Entry = std::conditional_t<kLayout == kWide32, uint32_t, uint16_t>;
   const int waves = blockDim.x / kWave;
-  const int wave = threadIdx.x / kWave, lane = lane_id();
-  const int first = forward_table_first(info);
-  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + forward_table_bytes(info)) + wave * info.hist_words;
-  stage_table(table_g + first, table, info.entries - first);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = lane_id();
+  unsigned char* lds;
+  uint32_t hist_base;  // byte address of the wave's histogram
+  if constexpr (kLayout == kPacked16) {
+    __shared__ __align__(16) unsigned char packed[kPackedTableBytes + kPackedWaves * kPackedHistBytes];
+    lds = packed;
+    hist_base = (uint32_t)(wave + kPackedTableBytes / kPackedHistBytes) * kPackedHistBytes;
+  } else {
+    extern __shared__ __align__(16) unsigned char smem[];
+    lds = smem;
+    hist_base = (uint32_t)(tab.bytes + wave * info.hist_words * 4);
+  }
+  uint32_t* hist = reinterpret_cast<uint32_t*>(lds + hist_base);
+  // counter word of an entry and the counter's bit position in it
+  auto counter = [&](uint32_t e) -> uint32_t* {
+    if constexpr (kLayout == kWide32) return reinterpret_cast<uint32_t*>(lds + hist_base + (e >> 16));
+    if constexpr (kLayout == kPacked16) return reinterpret_cast<uint32_t*>(lds + (hist_base | (e & ~3u)));
+    return reinterpret_cast<uint32_t*>(lds + hist_base + (e & ~3u));
+  };
+  // the low five bits are the shift; v_lshlrev and v_bfe read only those
+  auto shift = [](uint32_t e) -> uint32_t {
+    if constexpr (kLayout == kWide32) return e;
+    return e << 3;
+  };
+  {
+    const uint2* src = reinterpret_cast<const uint2*>(tab.data);
+    uint2* dst = reinterpret_cast<uint2*>(lds);
+    for (int i = threadIdx.x; i < tab.bytes / 8; i += blockDim.x) dst[i] = src[i];
+  }
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
 
   const int m = 2 * n;
   const int width = info.width;
-  const int offset = info.offset - first;  // staged index of (dr, dc) = key_b - key_a + offset
+  constexpr int unit = (int)sizeof(Entry);
+  const int span = n - 1;
+  // byte offset of the entry of (dr, dc) = key_b - key_a + zero
+  const int zero = unit * (span * tab.row_stride + span * tab.col_stride);
   for (long long s = (long long)blockIdx.x * waves + wave; s < count; s += (long long)gridDim.x * waves) {
     int key[CH];
     int outside = 0, coincide = 0;
@@ -414,58 +460,51 @@ __global__ void __launch_bounds__(1024) score_forward_kernel(const int32_t* __re
       continue;
     }
 
-    // key2: byte offset of a key in the staged table (2-byte entries)
-    int key2[CH], counts[CH], own[CH];
+    // at[c]: byte offset of the lane's point in the staged table's layout
+    int at[CH], counts[CH], own[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      key2[c] = 2 * key[c];
+      const int k = lane + kWave * c;
+      at[c] = unit * ((k >> 1) * tab.row_stride + (key[c] - (k >> 1) * width) * tab.col_stride);
       counts[c] = own[c] = 0;
     }
-    const bool tail_on = lane + kWave * (CH - 1) < m;
-    const unsigned char* table_bytes = reinterpret_cast<const unsigned char*>(table);
-    // Lanes without a forward slot look up the zero vector's entry (one
+    // Lanes without an earlier slot look up the zero vector's entry (one
     // broadcast address), read back and clear its word and skip only the
     // increment: the zero vector's byte stays 0, so their read-back adds 0
     // and no LDS step waits behind a branch.
-    const unsigned char* idle_entry = table_bytes + 2 * offset;
-    auto on = [&](int ca, int c, int l) {
-      bool v = (c == ca) ? lane > l : true;
-      if (c == CH - 1) v = v && tail_on;
-      return v;
-    };
-    // One anchor (slot 64 ca + l) is: lookup of the forward slots'
+    const unsigned char* idle_entry = lds + zero;
+    // One anchor (slot 64 ca + l) is: lookup of the earlier slots'
     // directions, histogram increments, read-back of each slot's count and
-    // clear.  Chunks before ca hold no forward slot; in chunk ca only the
-    // lanes past l do.  The next anchor's lookups (table only) issue together
-    // with this anchor's read-backs and clears, so each anchor waits on LDS
-    // once.  The shift (id << 3) is used unmasked: the shifts and bit-field
-    // extracts read its low five bits.
-    auto lookup = [&](int ca, int l, uint32_t(&id)[CH]) {
-      const unsigned char* from = table_bytes + (2 * offset - __builtin_amdgcn_readlane(key2[ca], l));
+    // clear.  Chunks before ca are all earlier slots, chunks after it none;
+    // in chunk ca the lanes before l are.  The next anchor's lookups (table
+    // only) issue together with this anchor's read-backs and clears, so each
+    // anchor waits on LDS once.
+    auto lookup = [&](int ca, int l, uint32_t(&e)[CH]) {
+      const unsigned char* from = lds + (zero - __builtin_amdgcn_readlane(at[ca], l));
 #pragma unroll
-      for (int c = ca; c < CH; ++c) {
-        const unsigned char* at = on(ca, c, l) ? from + key2[c] : idle_entry;
-        id[c] = *reinterpret_cast<const uint16_t*>(at);
+      for (int c = 0; c <= ca; ++c) {
+        const unsigned char* entry = (c < ca || lane < l) ? from + at[c] : idle_entry;
+        e[c] = *reinterpret_cast<const Entry*>(entry);
       }
     };
-    auto increment = [&](int ca, int l, const uint32_t(&id)[CH]) {
+    auto increment = [&](int ca, int l, const uint32_t(&e)[CH]) {
 #pragma unroll
-      for (int c = ca; c < CH; ++c)
-        if (on(ca, c, l)) atomicAdd(&hist[id[c] >> 2], 1u << (id[c] << 3));
+      for (int c = 0; c <= ca; ++c)
+        if (c < ca || lane < l) atomicAdd(counter(e[c]), 1u << (shift(e[c]) & 31u));
       wave_sync();
     };
-    auto collect = [&](int ca, const uint32_t(&id)[CH], uint32_t(&word)[CH]) {
+    auto collect = [&](int ca, const uint32_t(&e)[CH], uint32_t(&word)[CH]) {
 #pragma unroll
-      for (int c = ca; c < CH; ++c) word[c] = hist[id[c] >> 2];
+      for (int c = 0; c <= ca; ++c) word[c] = *counter(e[c]);
 #pragma unroll
-      for (int c = ca; c < CH; ++c) hist[id[c] >> 2] = 0u;
+      for (int c = 0; c <= ca; ++c) *counter(e[c]) = 0u;
     };
     // sum of the lanes' counts; each slot keeps its own running sum
-    auto accumulate = [&](int ca, const uint32_t(&id)[CH], const uint32_t(&word)[CH]) {
+    auto accumulate = [&](int ca, const uint32_t(&e)[CH], const uint32_t(&word)[CH]) {
       uint32_t part = 0;
 #pragma unroll
-      for (int c = ca; c < CH; ++c) {
-        const uint32_t cnt = __builtin_amdgcn_ubfe(word[c], id[c] << 3, 8);
+      for (int c = 0; c <= ca; ++c) {
+        const uint32_t cnt = __builtin_amdgcn_ubfe(word[c], shift(e[c]), 8);
         counts[c] += (int)cnt;
         part += cnt;
       }
@@ -473,44 +512,45 @@ __global__ void __launch_bounds__(1024) score_forward_kernel(const int32_t* __re
     };
 #pragma unroll
     for (int ca = 0; ca < CH; ++ca) {
-      // the last slot has no forward slots
-      const int limit = min(kWave, m - 1 - kWave * ca);
-      uint32_t id0[CH], id1[CH], word0[CH], word1[CH];
-      if (limit > 0) lookup(ca, 0, id0);
+      // slot 0 has no earlier slots
+      const int begin = (ca == 0) ? 1 : 0;
+      const int limit = min(kWave, m - kWave * ca);
+      uint32_t e0[CH], e1[CH], word0[CH], word1[CH];
+      if (begin < limit) lookup(ca, begin, e0);
       wave_sync();
-      for (int l = 0; l < limit; l += 2) {
-        increment(ca, l, id0);
-        collect(ca, id0, word0);
-        if (l + 1 < limit) lookup(ca, l + 1, id1);
+      for (int l = begin; l < limit; l += 2) {
+        increment(ca, l, e0);
+        collect(ca, e0, word0);
+        if (l + 1 < limit) lookup(ca, l + 1, e1);
         wave_sync();
-        const uint32_t p0 = accumulate(ca, id0, word0);
+        const uint32_t p0 = accumulate(ca, e0, word0);
         uint32_t p1 = 0;
         if (l + 1 < limit) {
-          increment(ca, l + 1, id1);
-          collect(ca, id1, word1);
-          if (l + 2 < limit) lookup(ca, l + 2, id0);
+          increment(ca, l + 1, e1);
+          collect(ca, e1, word1);
+          if (l + 2 < limit) lookup(ca, l + 2, e0);
           wave_sync();
-          p1 = accumulate(ca, id1, word1);
+          p1 = accumulate(ca, e1, word1);
         }
         // two anchors per reduction: an anchor's sum of counts is at most
-        // (m - 1)^2 < 2^16, so they share a word.  Anchor a has m - 1 - a
-        // forward slots; its first-slot triples are (sum - (m - 1 - a)) / 2.
+        // (m - 1)^2 < 2^16, so they share a word.  Anchor a has a earlier
+        // slots; its last-slot triples are (sum - a) / 2.
         const uint32_t total = wave_sum_dpp(p0 | (p1 << 16));
         const int a = kWave * ca + l;
-        const int first_own = (int)((total & 0xFFFFu) - (uint32_t)(m - 1 - a)) >> 1;
-        const int second_own = (int)((total >> 16) - (uint32_t)(m - 2 - a)) >> 1;
+        const int first_own = (int)((total & 0xFFFFu) - (uint32_t)a) >> 1;
+        const int second_own = (int)((total >> 16) - (uint32_t)(a + 1)) >> 1;
         own[ca] = (lane == l) ? first_own : own[ca];
         own[ca] = (lane == l + 1) ? second_own : own[ca];
       }
     }
-    // slot k was a forward slot of the anchors 0 .. k-1, and each of them
+    // slot k was an earlier slot of the anchors k+1 .. m-1, and each of them
     // adds its count, which includes k itself
     int sum = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
       if (k < m) {
-        const int b = counts[c] - k + own[c];
+        const int b = counts[c] - (m - 1 - k) + own[c];
         sum += b;
         if (badness) badness[s * m + k] = b;
       }
@@ -520,9 +560,59 @@ __global__ void __launch_bounds__(1024) score_forward_kernel(const int32_t* __re
   }
 }
 
+// Column stride of the wide layout: at least n, and 16 (mod 32).
+inline int wide_stride(int n) { return ((n + 15) / 32) * 32 + 16; }
+
+// Resident wide-layout tables (built from the direction ids on first use).
+static int wide_table(int n, const DirTableInfo& info, OneSidedTable* out) {
+  struct Wide {
+    int device, n;
+    uint32_t* data;
+    int bytes;
+  };
+  static std::mutex mutex;
+  static std::vector<Wide> tables;
+  int device = 0;
+  int st = hip_status(hipGetDevice(&device), "hipGetDevice");
+  if (st) return st;
+  const int stride = wide_stride(n), w = info.width, span = n - 1;
+  std::lock_guard<std::mutex> lock(mutex);
+  for (const auto& t : tables) {
+    if (t.device == device && t.n == n) {
+      *out = OneSidedTable{t.data, t.bytes, 1, stride};
+      return N3_OK;
+    }
+  }
+  std::vector<uint16_t> host_ids;
+  DirTableInfo built{};
+  st = build_dirtable(n, host_ids, built);
+  if (st) return st;
+  const int words = stride * w;
+  std::vector<uint32_t> host(((size_t)words + 3) / 4 * 4, 0u);
+  for (int dr = 0; dr <= span; ++dr) {      // rows dr - span = -(n-1) .. 0
+    for (int dc = 0; dc < w; ++dc) {
+      const uint32_t id = host_ids[(size_t)dr * w + dc];
+      host[(size_t)dr + (size_t)stride * dc] = ((id & ~3u) << 16) | ((id & 3u) << 3);
+    }
+  }
+  Wide t{device, n, nullptr, (int)(host.size() * 4)};
+  st = hip_status(hipMalloc(&t.data, host.size() * 4), "hipMalloc(wide table)");
+  if (st) return st;
+  st = hip_status(hipMemcpy(t.data, host.data(), host.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(wide table)");
+  if (st) {
+    (void)hipFree(t.data);
+    return st;
+  }
+  tables.push_back(t);
+  *out = OneSidedTable{t.data, t.bytes, 1, stride};
+  return N3_OK;
+}
+
+// Layout choice: the wide table where it leaves room for 16 histograms
+// (n <= 75), else the packed layout where it fits (n <= 80), else kHalf16.
 template <int CH>
-static int launch_forward(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
-                          const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
+static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
+                           const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
   hipDeviceProp_t prop;
   int device = 0;
   (void)hipGetDevice(&device);
@@ -530,19 +620,41 @@ static int launch_forward(const int32_t* p, const int32_t* q, int n, int64_t cou
   if (st) return st;
   int lds_limit = (int)prop.sharedMemPerBlock;
   if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
-  const int table_bytes = forward_table_bytes(info);
   const int hist_bytes = info.hist_words * 4;
-  int waves = (lds_limit - table_bytes) / hist_bytes;
-  if (waves > 16) waves = 16;
-  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
-  const size_t lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
+  const int wide_bytes = (wide_stride(n) * info.width * 4 + 15) / 16 * 16;
+  OneSidedTable tab{table, (n * info.width * 2 + 15) / 16 * 16, info.width, 1};
+  int layout = kHalf16;
+  if (wide_bytes + kPackedWaves * hist_bytes <= lds_limit) {
+    layout = kWide32;
+    st = wide_table(n, info, &tab);
+    if (st) return st;
+  } else if (tab.bytes <= kPackedTableBytes && hist_bytes <= kPackedHistBytes &&
+             lds_limit >= kPackedTableBytes + kPackedWaves * kPackedHistBytes) {
+    layout = kPacked16;
+  }
+  int waves = kPackedWaves;
+  size_t lds = 0;
+  if (layout != kPacked16) {
+    waves = (lds_limit - tab.bytes) / hist_bytes;
+    if (waves > 16) waves = 16;
+    if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
+    lds = (size_t)tab.bytes + (size_t)waves * hist_bytes;
+  }
   long long blocks = ((long long)count + waves - 1) / waves;
   const long long max_blocks = (long long)prop.multiProcessorCount;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(score_forward_kernel<CH>, dim3((unsigned)blocks), dim3(waves * kWave), lds, stream, p, q, n,
-                     (long long)count, scores, badness, table, info, status);
-  return hip_status(hipGetLastError(), "score_forward_kernel launch");
+  const dim3 grid((unsigned)blocks), block(waves * kWave);
+  if (layout == kPacked16)
+    hipLaunchKernelGGL((score_onesided_kernel<CH, kPacked16>), grid, block, 0, stream, p, q, n, (long long)count,
+                       scores, badness, tab, info, status);
+  else if (layout == kWide32)
+    hipLaunchKernelGGL((score_onesided_kernel<CH, kWide32>), grid, block, lds, stream, p, q, n, (long long)count,
+                       scores, badness, tab, info, status);
+  else
+    hipLaunchKernelGGL((score_onesided_kernel<CH, kHalf16>), grid, block, lds, stream, p, q, n, (long long)count,
+                       scores, badness, tab, info, status);
+  return hip_status(hipGetLastError(), "score_onesided_kernel launch");
 }
 
 template <int CH>
@@ -632,10 +744,11 @@ int score_population_impl(const int32_t* p, const int32_t* q, int n, int64_t cou
     st = device_dirtable(n, &table, &info);
     if (st) return st;
     switch (ch) {
-      case 1: st = launch_forward<1>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      case 2: st = launch_forward<2>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      case 3: st = launch_forward<3>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      default: st = launch_forward<4>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      // one chunk: the two-sided histogram needs fewer LDS steps
+      case 1: st = launch_slope<1>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 2: st = launch_onesided<2>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 3: st = launch_onesided<3>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      default: st = launch_onesided<4>(p, q, n, count, scores, badness, table, info, status, stream); break;
     }
   } else {
     return set_error(N3_EINVAL, "unknown scoring method %d", method);

@@ -31,7 +31,8 @@ def test_generator_matches_oracle(pkg, oracle, dev, n, count, seed):
 
 @pytest.mark.parametrize("method", [0, 1, 2])
 @pytest.mark.parametrize("n,count,seed", [(2, 8, 1), (3, 16, 9), (7, 64, 2), (32, 256, 0), (33, 64, 3),
-                                          (64, 64, 4), (75, 96, 5), (83, 32, 7), (100, 24, 6)])
+                                          (48, 64, 8), (64, 64, 4), (65, 40, 10), (75, 96, 5), (78, 48, 11),
+                                          (80, 32, 12), (83, 32, 7), (97, 16, 13), (100, 24, 6)])
 def test_scores_and_badness_match_oracle(pkg, oracle, dev, method, n, count, seed):
     hp, hq = oracle.generate_states(seed, n, count)
     scores, bad = _score(pkg, hp, hq, dev, method)
@@ -78,7 +79,7 @@ def test_coincident_points_and_non_permutations(pkg, oracle, dev):
     rows that are not permutations are counted exactly as the reference's
     determinant loop counts them."""
     rng = np.random.default_rng(11)
-    for n in (4, 9, 31, 75, 100):
+    for n in (4, 9, 31, 33, 75, 78, 100):
         p = rng.integers(0, n, size=(20, n)).astype(np.int32)
         q = rng.integers(0, n, size=(20, n)).astype(np.int32)
         q[:, : n // 3] = p[:, : n // 3]
@@ -87,6 +88,30 @@ def test_coincident_points_and_non_permutations(pkg, oracle, dev):
             ref_scores, ref_bad = oracle.score_population(p, q)
             assert np.array_equal(scores, ref_scores)
             assert np.array_equal(bad, ref_bad)
+
+
+@pytest.mark.parametrize("n", [33, 64, 75, 78, 83, 100])
+def test_lattice_states_with_long_lines(pkg, oracle, dev, n):
+    """States built from lines, P[r] = a r + b and Q[r] = c r + d (mod n),
+    without coincident points: up to n points lie on one line and most
+    pairs are collinear with others, so the histogram counters reach their
+    largest values (n - 1 earlier points on a line)."""
+    r = np.arange(n)
+    rows = []
+    for a, b, c, d in [(1, 0, 1, 1), (1, 0, n - 1, 0), (2, 0, 2, 1), (1, 0, 3, 0), (0, 0, 1, 0), (1, 5, 1, 7)]:
+        pr, qr = (a * r + b) % n, (c * r + d) % n
+        if np.any(pr == qr):
+            qr = (qr + 1 + (n // 2)) % n
+            if np.any(pr == qr):
+                continue
+        rows.append((pr, qr))
+    p = np.stack([x for x, _ in rows]).astype(np.int32)
+    q = np.stack([y for _, y in rows]).astype(np.int32)
+    for method in (0, 2):
+        scores, bad = _score(pkg, p, q, dev, method)
+        ref_scores, ref_bad = oracle.score_population(p, q)
+        assert np.array_equal(scores, ref_scores)
+        assert np.array_equal(bad, ref_bad)
 
 
 def test_out_of_range_columns_raise(pkg, dev):

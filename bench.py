@@ -26,6 +26,8 @@ N = 75
 POPULATION = 65536
 SEED = 0
 PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, int32 lane-ops
+# LDS: 128 bytes per clock per CU, i.e. 32 conflict-free 4-byte lane accesses
+PEAK_LDS_TACCESS = 256 * 32 * 2.4e9 / 1e12
 
 
 def log(msg: str) -> None:
@@ -33,12 +35,19 @@ def log(msg: str) -> None:
 
 
 def slope_ops_per_state(n: int) -> int:
-    """Algorithmic work of the direction-histogram scorer: for every anchor a
-    and point b (m^2 ordered pairs, a == b included as the zero bin): key
-    difference, table lookup, histogram increment, count read-back,
-    accumulate -- 5 integer operations per pair."""
+    """Algorithmic work of the one-sided direction-histogram scorer (the
+    kernel for 33 <= n <= 100): for every slot pair b < a, key difference,
+    table lookup, histogram increment, count read-back, counter clear and
+    accumulate -- 6 integer operations per unordered pair."""
     m = 2 * n
-    return 5 * m * m
+    return 6 * (m * (m - 1) // 2)
+
+
+def slope_lds_accesses_per_state(n: int) -> int:
+    """LDS lane accesses of the same kernel: lookup, increment, read-back and
+    clear per unordered slot pair."""
+    m = 2 * n
+    return 4 * (m * (m - 1) // 2)
 
 
 def determinant_ops_per_state(n: int) -> int:
@@ -49,7 +58,7 @@ def determinant_ops_per_state(n: int) -> int:
 
 def traffic_from_profile() -> float | None:
     """HBM bytes per scoring launch from the committed rocprofv3 PMC summary."""
-    path = ROOT / "profiles" / "score_slope_n75_P65536_pmc.txt"
+    path = ROOT / "profiles" / "score_onesided_n75_P65536_pmc.txt"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -316,6 +325,12 @@ def main() -> int:
                 "frac": achieved / PEAK_VALU_TOPS, "traffic": traffic_from_profile(),
                 "ops_per_state": ops, "kernel_ms": kernel_ms,
                 "triple_tests_per_s": POPULATION * (2 * N) * (2 * N - 1) * (2 * N - 2) / 6 / (kernel_ms * 1e-3)}
+    if args.method != 1:
+        # the unit that binds the kernel: LDS lane accesses (bank conflicts
+        # make most of the gap to the conflict-free peak)
+        lds = POPULATION * slope_lds_accesses_per_state(N) / (kernel_ms * 1e-3) / 1e12
+        roofline["lds"] = {"achieved": lds, "peak": PEAK_LDS_TACCESS, "unit": "T accesses/s",
+                           "frac": lds / PEAK_LDS_TACCESS, "accesses_per_state": slope_lds_accesses_per_state(N)}
 
     extra = {}
     legs = set() if args.no_temper else {x.strip() for x in args.legs.split(",") if x.strip()}

@@ -136,9 +136,19 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
   };
   build_cdf();
 
+  // Philox outputs of 64 consecutive steps, one step per lane, computed on
+  // the VALU in one pass and read back with readlane (the per-step counters,
+  // so the same stream as one draw per step)
+  uint32_t learn_r[4];
+  int batch = -kWave;  // first step of the current batch
   for (int step = 0; step < a.steps; ++step) {
+    if (step - batch >= kWave) {
+      batch = step;
+      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagLearn, a.seed_lo, a.seed_hi, learn_r);
+    }
     uint32_t r[4];
-    philox4x32_10((uint32_t)step, sweep, (uint32_t)slot, kTagLearn, a.seed_lo, a.seed_hi, r);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = (uint32_t)__builtin_amdgcn_readlane((int)learn_r[i], step - batch);
     int u, v;
     if (r[0] < kUniformShare) {
       u = (int)mulhi32(r[1], (uint32_t)m);

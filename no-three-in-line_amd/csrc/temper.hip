@@ -85,15 +85,28 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     if (first + (unsigned long long)a.steps <= (unsigned long long)a.log_capacity) log_base = a.log + 6 * first;
   }
 
+  // Philox outputs of 64 consecutive steps, one step per lane, computed on
+  // the VALU in one pass and read back with readlane: the counters are the
+  // per-step ones, so the stream is the same as one draw per step, and the
+  // scalar unit is left to the proposal bookkeeping
+  uint32_t move_r[4], bias_r[4] = {0u, 0u, 0u, 0u};
+  int batch = -kWave;  // first step of the current batch
   for (int step = 0; step < a.steps; ++step) {
+    if (step - batch >= kWave) {
+      batch = step;
+      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagMove, a.seed_lo, a.seed_hi, move_r);
+      if (a.bad_bias > 0)
+        philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagBias, a.seed_lo, a.seed_hi, bias_r);
+    }
+    const int draw = step - batch;
     uint32_t r[4];
-    philox4x32_10((uint32_t)step, sweep, (uint32_t)slot, kTagMove, a.seed_lo, a.seed_hi, r);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = (uint32_t)__builtin_amdgcn_readlane((int)move_r[i], draw);
     int u = (int)mulhi32(r[0], (uint32_t)m);
     if (a.bad_bias > 0 && score > 0 && (int)mulhi32(r[3], 100u) < a.bad_bias) {
       // tempering.py:51-55: u drawn in proportion to badness
-      uint32_t rb[4];
-      philox4x32_10((uint32_t)step, sweep, (uint32_t)slot, kTagBias, a.seed_lo, a.seed_hi, rb);
-      const int target = (int)mulhi32(rb[0], (uint32_t)(3 * score));
+      const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readlane((int)bias_r[0], draw);
+      const int target = (int)mulhi32(rb0, (uint32_t)(3 * score));
       int below = 0, carry = 0;
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
@@ -131,43 +144,65 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       const int k = lane + kWave * c;
       others[c] = (k < m) && k != u && k != v;
       idu[c] = table[key[c] - key_u + offset];
-      on_uv[c] = __ballot(others[c] && idu[c] == id_uv);
-      joint_old += __popcll(on_uv[c]);
     }
-    // histograms seen from u' and v' in the moved state
+    // histograms seen from u' and v' in the moved state.  The rejection path
+    // needs only bad'(u') and bad'(v') = sum_d C(cnt_d, 2), the sums of the
+    // ranks the increments return, so both histograms are built and cleared
+    // back to back with one wait; the per-slot counts are read only when the
+    // move is accepted.
     const int key_u2 = ru * width + cv, key_v2 = rv * width + cu;
     const int id_u2v2 = table[key_v2 - key_u2 + offset];
-    int keys[CH], cnt_u2[CH], cnt_v2[CH], id[CH];
-    bool include[CH];
+    int idu2[CH], idv2[CH];
+    bool inc_u2[CH], inc_v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      include[c] = (k < m) && k != u;
-      keys[c] = (k == v) ? key_v2 : key[c];
+      inc_u2[c] = (k < m) && k != u;
+      inc_v2[c] = (k < m) && k != v;
+      idu2[c] = table[((k == v) ? key_v2 : key[c]) - key_u2 + offset];
+      idv2[c] = table[((k == u) ? key_u2 : key[c]) - key_v2 + offset];
     }
-    direction_counts<CH>(table, hist, trash, offset, key_u2, keys, include, cnt_u2, id);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      on_uv[c] = __ballot(others[c] && idu[c] == id_uv);
+      joint_old += __popcll(on_uv[c]);
+    }
+    uint32_t word_u2[CH], word_v2[CH], old_u2[CH], old_v2[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      word_u2[c] = inc_u2[c] ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
+      old_u2[c] = atomicAdd(&hist[word_u2[c]], 1u << ((uint32_t)idu2[c] << 3));
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < CH; ++c) hist[word_u2[c]] = 0u;
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      word_v2[c] = inc_v2[c] ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
+      old_v2[c] = atomicAdd(&hist[word_v2[c]], 1u << ((uint32_t)idv2[c] << 3));
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < CH; ++c) hist[word_v2[c]] = 0u;
+    wave_sync();
+    // keep the rank extraction below every LDS step above, so the
+    // increments are not held up by waits for earlier ranks
+    __builtin_amdgcn_sched_barrier(0);
     uint32_t part = 0;
     int joint_new = 0;
     uint64_t on_u2v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      if (include[c]) part += (uint32_t)(cnt_u2[c] - 1);
-      on_u2v2[c] = __ballot(others[c] && id[c] == id_u2v2);
+      // a lane's trash word serves all its chunks: excluded lanes add 0
+      if (inc_u2[c]) part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8);
+      if (inc_v2[c]) part += __builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16;
+      on_u2v2[c] = __ballot(others[c] && idu2[c] == id_u2v2);
       joint_new += __popcll(on_u2v2[c]);
     }
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int k = lane + kWave * c;
-      include[c] = (k < m) && k != v;
-      keys[c] = (k == u) ? key_u2 : key[c];
-    }
-    direction_counts<CH>(table, hist, trash, offset, key_v2, keys, include, cnt_v2, id);
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-      if (include[c]) part += (uint32_t)(cnt_v2[c] - 1) << 16;
-    // twice bad'(u') in the low half, twice bad'(v') in the high half
+    // bad'(u') in the low half, bad'(v') in the high half (each < 2^16)
     const uint32_t packed = wave_sum_dpp(part);
-    const int bad_u2 = (int)((packed & 0xFFFFu) >> 1), bad_v2 = (int)((packed >> 16) >> 1);
+    const int bad_u2 = (int)(packed & 0xFFFFu), bad_v2 = (int)(packed >> 16);
     const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
     const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
 
@@ -187,8 +222,12 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     }
     if (!accept) continue;
 
-    // badness of the unmoved slots: histograms seen from u and v now
-    int cnt_u[CH], cnt_v[CH];
+    // badness of the unmoved slots: per-slot counts seen from u', v' in the
+    // moved state and from u, v now
+    int cnt_u2[CH], cnt_v2[CH], cnt_u[CH], cnt_v[CH], id[CH];
+    bool include[CH];
+    direction_counts_known<CH>(hist, trash, idu2, inc_u2, cnt_u2);
+    direction_counts_known<CH>(hist, trash, idv2, inc_v2, cnt_v2);
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
     direction_counts_known<CH>(hist, trash, idu, include, cnt_u);

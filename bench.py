@@ -262,6 +262,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-temper", action="store_true", help="skip the tempering, batched and learned legs")
+    ap.add_argument("--temper-sweeps", type=int, default=10000,
+                    help="sweeps of the tempering leg (config 3 is 10,000; fewer for counter profiles)")
     ap.add_argument("--legs", default="temper,batched,learned,near,search",
                     help="comma list of the extra legs to run: temper, batched, learned, near, search")
     ap.add_argument("--ab-seconds", type=float, default=20.0, help="wall-clock budget of each A/B side")
@@ -340,7 +342,7 @@ def main() -> int:
         return 2
     if "temper" in legs:
         log("bench: tempering rate")
-        extra["tempering_n75_64x64"] = tempering_rate(pkg, torch)
+        extra["tempering_n75_64x64"] = tempering_rate(pkg, torch, sweeps=args.temper_sweeps)
     if "batched" in legs:
         log("bench: batched uneven sizes")
         extra["batched_uneven"] = batched_rate(pkg, torch)

@@ -56,7 +56,21 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     bad[c] = k < m ? a.badness[base + k] : 0;
     ref[c] = k < m ? a.reference[k] : -1;
   }
-  // lanes past the last slot hold slot 0's key (always a valid point)
+  // both slots of a row, packed per lane: current columns (bytes 0, 1) and
+  // reference columns (bytes 2, 3) of slots 2r and 2r+1; one readlane per
+  // chunk gives all four for a row
+  int row_word[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    const int col = key[c] - (k >> 1) * width, partner_col = __shfl_xor(col, 1, kWave);
+    const int partner_ref = __shfl_xor(ref[c], 1, kWave);
+    const int col0 = (k & 1) ? partner_col : col, col1 = (k & 1) ? col : partner_col;
+    const int ref0 = (k & 1) ? partner_ref : ref[c], ref1 = (k & 1) ? ref[c] : partner_ref;
+    row_word[c] = (col0 & 0xFF) | (col1 & 0xFF) << 8 | (ref0 & 0xFF) << 16 | (ref1 & 0xFF) << 24;
+  }
+  // lanes past the last slot hold slot 0's key (always a valid point); in
+  // the histograms they update their own trash word, in the last chunk only
   {
     const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
 #pragma unroll
@@ -64,6 +78,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       if (lane + kWave * c >= m) key[c] = key0;
   }
   const int trash = trash_word(info, lane);
+  const bool tail_in = lane + kWave * (CH - 1) < m;
   int score = a.scores[slot];
   int removed = a.removed[slot];
   int best = a.best_scores[slot];
@@ -94,9 +109,12 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
   for (int step = 0; step < a.steps; ++step) {
     if (step - batch >= kWave) {
       batch = step;
-      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagMove, a.seed_lo, a.seed_hi, move_r);
-      if (a.bad_bias > 0)
-        philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagBias, a.seed_lo, a.seed_hi, bias_r);
+      // opaque copies of the key: its round schedule is derived here, once
+      // per batch, instead of being kept live in scalar registers all loop
+      uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+      asm volatile("" : "+s"(k0), "+s"(k1));
+      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagMove, k0, k1, move_r);
+      if (a.bad_bias > 0) philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagBias, k0, k1, bias_r);
     }
     const int draw = step - batch;
     uint32_t r[4];
@@ -120,16 +138,16 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     }
     const int v = (int)mulhi32(r[1], (uint32_t)m);
     const int ru = u >> 1, rv = v >> 1;
-    const int key_u = slot_value<CH>(key, u), key_v = slot_value<CH>(key, v);
-    const int cu = key_u - ru * width, cv = key_v - rv * width;
-    const int cu_other = slot_value<CH>(key, u ^ 1) - ru * width;
-    const int cv_other = slot_value<CH>(key, v ^ 1) - rv * width;
+    const uint32_t word_ru = (uint32_t)slot_value<CH>(row_word, u & ~1);
+    const uint32_t word_rv = (uint32_t)slot_value<CH>(row_word, v & ~1);
+    const int cu = (int)((word_ru >> (8 * (u & 1))) & 0xFF), cu_other = (int)((word_ru >> (8 * (~u & 1))) & 0xFF);
+    const int cv = (int)((word_rv >> (8 * (v & 1))) & 0xFF), cv_other = (int)((word_rv >> (8 * (~v & 1))) & 0xFF);
+    const int key_u = ru * width + cu, key_v = rv * width + cv;
     const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
     if (!valid) continue;
     valid_moves += 1;
-    int ref_u0, ref_u1, ref_v0, ref_v1;
-    slot_pair<CH>(ref, ru, ref_u0, ref_u1);
-    slot_pair<CH>(ref, rv, ref_v0, ref_v1);
+    const int ref_u0 = (int)((word_ru >> 16) & 0xFF), ref_u1 = (int)(word_ru >> 24);
+    const int ref_v0 = (int)((word_rv >> 16) & 0xFF), ref_v1 = (int)(word_rv >> 24);
     const int dd = (cu == ref_u0 || cu == ref_u1) + (cv == ref_v0 || cv == ref_v1) - (cv == ref_u0 || cv == ref_u1) -
                    (cu == ref_v0 || cu == ref_v1);
 
@@ -137,7 +155,6 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     const int id_uv = table[key_v - key_u + offset];
     int idu[CH];
     bool others[CH];
-    uint64_t on_uv[CH];
     int joint_old = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
@@ -152,25 +169,25 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     // move is accepted.
     const int key_u2 = ru * width + cv, key_v2 = rv * width + cu;
     const int id_u2v2 = table[key_v2 - key_u2 + offset];
+    // the moved slot itself looks up the zero vector (its rank is 0): the
+    // lanes need no exclusion mask except past the last slot
     int idu2[CH], idv2[CH];
-    bool inc_u2[CH], inc_v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      inc_u2[c] = (k < m) && k != u;
-      inc_v2[c] = (k < m) && k != v;
-      idu2[c] = table[((k == v) ? key_v2 : key[c]) - key_u2 + offset];
-      idv2[c] = table[((k == u) ? key_u2 : key[c]) - key_v2 + offset];
+      const int moved = (k == v) ? key_v2 : ((k == u) ? key_u2 : key[c]);
+      idu2[c] = table[moved - key_u2 + offset];
+      idv2[c] = table[moved - key_v2 + offset];
     }
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      on_uv[c] = __ballot(others[c] && idu[c] == id_uv);
-      joint_old += __popcll(on_uv[c]);
+      // the masks are not kept: the accepted path compares again per lane
+      joint_old += __popcll(__ballot(others[c] && idu[c] == id_uv));
     }
     uint32_t word_u2[CH], word_v2[CH], old_u2[CH], old_v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      word_u2[c] = inc_u2[c] ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
+      word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
       old_u2[c] = atomicAdd(&hist[word_u2[c]], 1u << ((uint32_t)idu2[c] << 3));
     }
     wave_sync();
@@ -179,7 +196,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     wave_sync();
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      word_v2[c] = inc_v2[c] ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
+      word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
       old_v2[c] = atomicAdd(&hist[word_v2[c]], 1u << ((uint32_t)idv2[c] << 3));
     }
     wave_sync();
@@ -191,14 +208,13 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     __builtin_amdgcn_sched_barrier(0);
     uint32_t part = 0;
     int joint_new = 0;
-    uint64_t on_u2v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      // a lane's trash word serves all its chunks: excluded lanes add 0
-      if (inc_u2[c]) part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8);
-      if (inc_v2[c]) part += __builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16;
-      on_u2v2[c] = __ballot(others[c] && idu2[c] == id_u2v2);
-      joint_new += __popcll(on_u2v2[c]);
+      // trash words (one per lane, last chunk only) and the zero vector of
+      // the moved slot return rank 0
+      part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
+              (__builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16);
+      joint_new += __popcll(__ballot(others[c] && idu2[c] == id_u2v2));
     }
     // bad'(u') in the low half, bad'(v') in the high half (each < 2^16)
     const uint32_t packed = wave_sum_dpp(part);
@@ -226,8 +242,12 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     // moved state and from u, v now
     int cnt_u2[CH], cnt_v2[CH], cnt_u[CH], cnt_v[CH], id[CH];
     bool include[CH];
-    direction_counts_known<CH>(hist, trash, idu2, inc_u2, cnt_u2);
-    direction_counts_known<CH>(hist, trash, idv2, inc_v2, cnt_v2);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
+    direction_counts_known<CH>(hist, trash, idu2, include, cnt_u2);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+    direction_counts_known<CH>(hist, trash, idv2, include, cnt_v2);
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
     direction_counts_known<CH>(hist, trash, idu, include, cnt_u);
@@ -237,8 +257,8 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      const int bit_old = (int)((on_uv[c] >> lane) & 1ull);
-      const int bit_new = (int)((on_u2v2[c] >> lane) & 1ull);
+      const int bit_old = idu[c] == id_uv ? 1 : 0;
+      const int bit_new = idu2[c] == id_u2v2 ? 1 : 0;
       if (others[c]) bad[c] += (cnt_u2[c] + cnt_v2[c] - bit_new) - (cnt_u[c] + cnt_v[c] - bit_old);
       if (k == u) {
         bad[c] = bad_u2;
@@ -248,6 +268,9 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
         bad[c] = bad_v2;
         key[c] = key_v2;
       }
+      // u now holds column cv, v column cu: both lanes of each row
+      if ((k >> 1) == ru) row_word[c] = (int)(((uint32_t)row_word[c] & ~(0xFFu << (8 * (u & 1)))) | ((uint32_t)cv << (8 * (u & 1))));
+      if ((k >> 1) == rv) row_word[c] = (int)(((uint32_t)row_word[c] & ~(0xFFu << (8 * (v & 1)))) | ((uint32_t)cu << (8 * (v & 1))));
     }
     score += delta;
     removed += dd;

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "runtime.h"
+#include "temper_internal.h"
 #include "walk.h"
 #include "wave_ops.h"
 
@@ -367,20 +368,25 @@ struct ExchangeArgs {
   const int8_t* sign;
   int k_exchange;
   uint32_t* counters;
+  // with `done` set, the last block to finish also advances counters[0] by
+  // d_sweep and counters[1] by d_round (see exchange_kernel)
+  uint32_t* done;
+  uint32_t d_sweep, d_round;
 };
 
 // One wavefront per candidate pair (a, a+1); pairs start at round % 2 inside
 // every ladder and do not overlap, so the swaps need no synchronisation.
-__device__ void exchange_pair(const ExchangeArgs& x, int wave) {
+// Returns whether the pair swapped.
+__device__ bool exchange_pair(const ExchangeArgs& x, int wave) {
   const int lane = lane_id();
   const uint32_t round = x.counters[1];
   const int per_ladder = x.ladder / 2;  // candidate pairs per ladder (upper bound)
   const int ladders = x.replicas / x.ladder;
-  if (wave >= ladders * per_ladder) return;
+  if (wave >= ladders * per_ladder) return false;
   const int ladder = wave / per_ladder;
   const int start = ladder * x.ladder;
   const int a = start + (int)(round & 1u) + 2 * (wave % per_ladder);
-  if (a >= start + x.ladder - 1) return;
+  if (a >= start + x.ladder - 1) return false;
   const int b = a + 1;
   const long long diff = (long long)x.scores[a] - (long long)x.scores[b];
   const int sg = x.sign[a];
@@ -394,7 +400,7 @@ __device__ void exchange_pair(const ExchangeArgs& x, int wave) {
     if (d > x.k_exchange - 1) d = x.k_exchange - 1;
     accept = r[0] < x.thr[(long long)a * x.k_exchange + d];
   }
-  if (!accept) return;
+  if (!accept) return false;
   const int m = 2 * x.n;
   for (int i = lane; i < m; i += kWave) {
     const long long ia = (long long)a * m + i, ib = (long long)b * m + i;
@@ -412,12 +418,32 @@ __device__ void exchange_pair(const ExchangeArgs& x, int wave) {
     const int32_t ra = x.removed[a];
     x.removed[a] = x.removed[b];
     x.removed[b] = ra;
-    atomicAdd(reinterpret_cast<unsigned long long*>(x.counters + 2), 1ull);
   }
+  return true;
 }
 
-__global__ void __launch_bounds__(256) exchange_kernel(ExchangeArgs x) {
-  exchange_pair(x, (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+// The block's swaps are counted with one atomic.  With x.done set, each block
+// then adds 1 to *done, and the block that brings it to gridDim.x -- after
+// every block has read the round -- advances the sweep and round counters
+// and resets *done: one launch per sweep instead of two counter kernels.
+__global__ void __launch_bounds__(1024) exchange_kernel(ExchangeArgs x) {
+  __shared__ uint32_t swaps;
+  if (threadIdx.x == 0) swaps = 0;
+  __syncthreads();
+  const bool swapped = exchange_pair(x, (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+  if (swapped && lane_id() == 0) atomicAdd(&swaps, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (swaps) atomicAdd(reinterpret_cast<unsigned long long*>(x.counters + 2), (unsigned long long)swaps);
+    if (x.done) {
+      __threadfence();
+      if (atomicAdd(x.done, 1u) == gridDim.x - 1) {
+        atomicAdd(&x.counters[0], x.d_sweep);
+        atomicAdd(&x.counters[1], x.d_round);
+        atomicExch(x.done, 0u);
+      }
+    }
+  }
 }
 
 // Device-side description of a batched run (one allocation per plan).
@@ -428,14 +454,35 @@ struct BatchTables {
   uint32_t* counters[N3_MAX_GROUPS];
   uint32_t round_step[N3_MAX_GROUPS];     // 1 if the group exchanges
   int count;
+  uint32_t done;  // blocks of the running exchange launch that finished
 };
 
-__global__ void __launch_bounds__(256) exchange_batched_kernel(const BatchTables* __restrict__ t) {
+// As exchange_kernel, for every group of a batched run: swaps are counted per
+// block and group, and the last block to finish advances every group's sweep
+// counter and, for exchanging groups, its round counter.
+__global__ void __launch_bounds__(1024) exchange_batched_kernel(BatchTables* __restrict__ t) {
+  __shared__ uint32_t swaps[N3_MAX_GROUPS];
+  if (threadIdx.x < N3_MAX_GROUPS) swaps[threadIdx.x] = 0;
+  __syncthreads();
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
   int g = 0;
   while (g + 1 < t->count && wave >= t->exchange_begin[g + 1]) ++g;
-  if (wave >= t->exchange_begin[t->count] || t->round_step[g] == 0) return;
-  exchange_pair(t->exchange[g], wave - t->exchange_begin[g]);
+  if (wave < t->exchange_begin[t->count] && t->round_step[g] != 0) {
+    if (exchange_pair(t->exchange[g], wave - t->exchange_begin[g]) && lane_id() == 0) atomicAdd(&swaps[g], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < t->count; ++i)
+      if (swaps[i]) atomicAdd(reinterpret_cast<unsigned long long*>(t->counters[i] + 2), (unsigned long long)swaps[i]);
+    __threadfence();
+    if (atomicAdd(&t->done, 1u) == gridDim.x - 1) {
+      for (int i = 0; i < t->count; ++i) {
+        atomicAdd(&t->counters[i][0], 1u);
+        atomicAdd(&t->counters[i][1], t->round_step[i]);
+      }
+      atomicExch(&t->done, 0u);
+    }
+  }
 }
 
 __global__ void bump_batched_kernel(const BatchTables* __restrict__ t) {
@@ -516,28 +563,33 @@ WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed) {
                   c->log, (long long)c->log_capacity, reinterpret_cast<unsigned long long*>(c->log_count)};
 }
 
+// The walk; with bump_sweep the sweep counter is advanced by a counter kernel
+// after it (a following fused exchange may advance it instead).
 static int enqueue_advance(const n3_chains* c, const WalkPlan& plan, int steps, int bad_bias, uint64_t seed,
-                           hipStream_t stream) {
+                           hipStream_t stream, bool bump_sweep = true) {
   hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream,
                      walk_args(c, steps, bad_bias, seed), plan.table, plan.info);
   int st = hip_status(hipGetLastError(), "walk_kernel launch");
-  if (st) return st;
+  if (st || !bump_sweep) return st;
   hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, 1u, 0u);
   return hip_status(hipGetLastError(), "bump_kernel launch");
 }
 
-int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream) {
-  ExchangeArgs x{c->n, c->replicas, ladder, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
-                 c->scores, c->removed_points, c->exchange_thresholds, c->exchange_sign, c->k_exchange, c->counters};
+int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream, uint32_t* done,
+                     uint32_t d_sweep) {
   const int pairs = (c->replicas / ladder) * (ladder / 2);
+  ExchangeArgs x{c->n, c->replicas, ladder, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
+                 c->scores, c->removed_points, c->exchange_thresholds, c->exchange_sign, c->k_exchange, c->counters,
+                 pairs > 0 ? done : nullptr, d_sweep, 1u};
   if (pairs > 0) {
-    const int threads = 256, per_block = threads / kWave;
+    const int threads = 1024, per_block = threads / kWave;  // few blocks: few counter atomics
     hipLaunchKernelGGL(exchange_kernel, dim3((unsigned)((pairs + per_block - 1) / per_block)), dim3(threads), 0, stream,
                        x);
     int st = hip_status(hipGetLastError(), "exchange_kernel launch");
     if (st) return st;
+    if (x.done) return N3_OK;
   }
-  hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, 0u, 1u);
+  hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, d_sweep, 1u);
   return hip_status(hipGetLastError(), "bump_kernel launch");
 }
 
@@ -570,21 +622,24 @@ struct GraphEntry {
   GraphKey key;
   hipGraphExec_t exec;
   hipGraph_t graph;
+  uint32_t* done;  // the fused exchange's last-block counter (zero between launches)
 };
 
 static std::mutex g_graph_mutex;
 static std::vector<GraphEntry> g_graphs;
 
+// A graph of `sweeps` x (walk, exchange): the exchange kernel advances both
+// counters through the entry's `done` word, so a sweep is two launches.
 static int capture_graph(const n3_chains* c, const WalkPlan& plan, int sweeps, int steps, int ladder, int bad_bias,
-                         uint64_t seed, hipGraph_t* graph, hipGraphExec_t* exec) {
+                         uint64_t seed, uint32_t* done, hipGraph_t* graph, hipGraphExec_t* exec) {
   hipStream_t cap = nullptr;
   int st = hip_status(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
   if (st) return st;
   st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (!st) {
     for (int i = 0; i < sweeps && !st; ++i) {
-      st = enqueue_advance(c, plan, steps, bad_bias, seed, cap);
-      if (!st && ladder > 1) st = enqueue_exchange(c, ladder, seed, cap);
+      st = enqueue_advance(c, plan, steps, bad_bias, seed, cap, ladder <= 1);
+      if (!st && ladder > 1) st = enqueue_exchange(c, ladder, seed, cap, done, 1u);
     }
     hipGraph_t g = nullptr;
     const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
@@ -674,7 +729,7 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
   plan->walk_blocks = (unsigned)blocks;
   plan->walk_threads = max_waves * kWave;
   plan->walk_lds = lds;
-  plan->exchange_blocks = (unsigned)((exchange_waves + 3) / 4);
+  plan->exchange_blocks = (unsigned)((exchange_waves + 15) / 16);  // 1024-thread blocks
   st = hip_status(hipMalloc(&plan->device, sizeof(BatchTables)), "hipMalloc(batch tables)");
   if (st) return st;
   st = hip_status(hipMemcpy(plan->device, &plan->host, sizeof(BatchTables), hipMemcpyHostToDevice),
@@ -694,9 +749,9 @@ static int enqueue_batched_sweep(const BatchPlan& plan, hipStream_t stream) {
   int st = hip_status(hipGetLastError(), "walk_batched_kernel launch");
   if (st) return st;
   if (plan.exchange_blocks > 0) {
-    hipLaunchKernelGGL(exchange_batched_kernel, dim3(plan.exchange_blocks), dim3(256), 0, stream, plan.device);
-    st = hip_status(hipGetLastError(), "exchange_batched_kernel launch");
-    if (st) return st;
+    // the exchange launch also advances the counters
+    hipLaunchKernelGGL(exchange_batched_kernel, dim3(plan.exchange_blocks), dim3(1024), 0, stream, plan.device);
+    return hip_status(hipGetLastError(), "exchange_batched_kernel launch");
   }
   hipLaunchKernelGGL(bump_batched_kernel, dim3(1), dim3(64), 0, stream, plan.device);
   return hip_status(hipGetLastError(), "bump_batched_kernel launch");
@@ -831,10 +886,17 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
     for (const auto& e : g_graphs)
       if (same_key(e.key, key)) exec = e.exec;
     if (!exec) {
-      hipGraph_t graph = nullptr;
-      st = capture_graph(chains, plan, per_graph, steps, ladder, bad_bias, seed, &graph, &exec);
+      uint32_t* done = nullptr;
+      st = hip_status(hipMalloc(&done, sizeof(uint32_t)), "hipMalloc(exchange counter)");
       if (st) return st;
-      g_graphs.push_back(GraphEntry{key, exec, graph});
+      st = hip_status(hipMemset(done, 0, sizeof(uint32_t)), "hipMemset(exchange counter)");
+      hipGraph_t graph = nullptr;
+      if (!st) st = capture_graph(chains, plan, per_graph, steps, ladder, bad_bias, seed, done, &graph, &exec);
+      if (st) {
+        (void)hipFree(done);
+        return st;
+      }
+      g_graphs.push_back(GraphEntry{key, exec, graph, done});
     }
   }
   for (int i = 0; i < full; ++i) {
@@ -853,6 +915,7 @@ extern "C" void n3_temper_release(void) {
   for (auto& e : g_graphs) {
     (void)hipGraphExecDestroy(e.exec);
     (void)hipGraphDestroy(e.graph);
+    (void)hipFree(e.done);
   }
   g_graphs.clear();
   for (auto& e : g_batches) {

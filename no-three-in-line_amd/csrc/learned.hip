@@ -85,6 +85,18 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     bad[c] = k < m ? a.badness[base + k] : 0;
     ref[c] = k < m ? a.reference[k] : -1;
   }
+  // both slots of a row, packed per lane: current columns (bytes 0, 1) and
+  // reference columns (bytes 2, 3) of slots 2r and 2r+1
+  int row_word[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int k = lane + kWave * c;
+    const int col = key[c] - (k >> 1) * width, partner_col = __shfl_xor(col, 1, kWave);
+    const int partner_ref = __shfl_xor(ref[c], 1, kWave);
+    const int col0 = (k & 1) ? partner_col : col, col1 = (k & 1) ? col : partner_col;
+    const int ref0 = (k & 1) ? partner_ref : ref[c], ref1 = (k & 1) ? ref[c] : partner_ref;
+    row_word[c] = (col0 & 0xFF) | (col1 & 0xFF) << 8 | (ref0 & 0xFF) << 16 | (ref1 & 0xFF) << 24;
+  }
   {
     const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
 #pragma unroll
@@ -92,6 +104,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
       if (lane + kWave * c >= m) key[c] = key0;
   }
   const int trash = trash_word(info, lane);
+  const bool tail_in = lane + kWave * (CH - 1) < m;
   int score = a.scores[slot];
   int removed = a.removed[slot];
   int best = a.best_scores[slot];
@@ -167,64 +180,71 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
       v = below_v < m - 1 ? below_v : m - 1;
     }
     const int ru = u >> 1, rv = v >> 1;
-    const int key_u = slot_value<CH>(key, u), key_v = slot_value<CH>(key, v);
-    const int cu = key_u - ru * width, cv = key_v - rv * width;
-    const int cu_other = slot_value<CH>(key, u ^ 1) - ru * width;
-    const int cv_other = slot_value<CH>(key, v ^ 1) - rv * width;
+    const uint32_t word_ru = (uint32_t)slot_value<CH>(row_word, u & ~1);
+    const uint32_t word_rv = (uint32_t)slot_value<CH>(row_word, v & ~1);
+    const int cu = (int)((word_ru >> (8 * (u & 1))) & 0xFF), cu_other = (int)((word_ru >> (8 * (~u & 1))) & 0xFF);
+    const int cv = (int)((word_rv >> (8 * (v & 1))) & 0xFF), cv_other = (int)((word_rv >> (8 * (~v & 1))) & 0xFF);
+    const int key_u = ru * width + cu, key_v = rv * width + cv;
     const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
     if (!valid) continue;
     valid_moves += 1;
-    int ref_u0, ref_u1, ref_v0, ref_v1;
-    slot_pair<CH>(ref, ru, ref_u0, ref_u1);
-    slot_pair<CH>(ref, rv, ref_v0, ref_v1);
+    const int ref_u0 = (int)((word_ru >> 16) & 0xFF), ref_u1 = (int)(word_ru >> 24);
+    const int ref_v0 = (int)((word_rv >> 16) & 0xFF), ref_v1 = (int)(word_rv >> 24);
     const int dd = (cu == ref_u0 || cu == ref_u1) + (cv == ref_v0 || cv == ref_v1) - (cv == ref_u0 || cv == ref_u1) -
                    (cu == ref_v0 || cu == ref_v1);
 
+    // bad'(u') and bad'(v') from the ranks the increments return (both
+    // histograms back to back, one wait); per-slot counts are read back only
+    // for the full evaluation.  The moved slot itself looks up the zero
+    // vector (rank 0), lanes past the last slot use their trash word.
     const int id_uv = table[key_v - key_u + offset];
-    int idu[CH];
+    const int key_u2 = ru * width + cv, key_v2 = rv * width + cu;
+    const int id_u2v2 = table[key_v2 - key_u2 + offset];
+    int idu[CH], idu2[CH], idv2[CH];
     bool others[CH];
-    uint64_t on_uv[CH];
-    int joint_old = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
       others[c] = (k < m) && k != u && k != v;
       idu[c] = table[key[c] - key_u + offset];
-      on_uv[c] = __ballot(others[c] && idu[c] == id_uv);
-      joint_old += __popcll(on_uv[c]);
+      const int moved = (k == v) ? key_v2 : ((k == u) ? key_u2 : key[c]);
+      idu2[c] = table[moved - key_u2 + offset];
+      idv2[c] = table[moved - key_v2 + offset];
     }
-    const int key_u2 = ru * width + cv, key_v2 = rv * width + cu;
-    const int id_u2v2 = table[key_v2 - key_u2 + offset];
-    int keys[CH], cnt_u2[CH], cnt_v2[CH], cnt_u[CH], cnt_v[CH], id[CH];
-    bool include[CH];
+    int joint_old = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) joint_old += __popcll(__ballot(others[c] && idu[c] == id_uv));
+    uint32_t word_u2[CH], word_v2[CH], old_u2[CH], old_v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const int k = lane + kWave * c;
-      include[c] = (k < m) && k != u;
-      keys[c] = (k == v) ? key_v2 : key[c];
+      word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
+      old_u2[c] = atomicAdd(&hist[word_u2[c]], 1u << ((uint32_t)idu2[c] << 3));
     }
-    direction_counts<CH>(table, hist, trash, offset, key_u2, keys, include, cnt_u2, id);
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < CH; ++c) hist[word_u2[c]] = 0u;
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
+      old_v2[c] = atomicAdd(&hist[word_v2[c]], 1u << ((uint32_t)idv2[c] << 3));
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < CH; ++c) hist[word_v2[c]] = 0u;
+    wave_sync();
+    // keep the rank extraction below every LDS step above
+    __builtin_amdgcn_sched_barrier(0);
     uint32_t part = 0;
     int joint_new = 0;
-    uint64_t on_u2v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      if (include[c]) part += (uint32_t)(cnt_u2[c] - 1);
-      on_u2v2[c] = __ballot(others[c] && id[c] == id_u2v2);
-      joint_new += __popcll(on_u2v2[c]);
+      part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
+              (__builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16);
+      joint_new += __popcll(__ballot(others[c] && idu2[c] == id_u2v2));
     }
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int k = lane + kWave * c;
-      include[c] = (k < m) && k != v;
-      keys[c] = (k == u) ? key_u2 : key[c];
-    }
-    direction_counts<CH>(table, hist, trash, offset, key_v2, keys, include, cnt_v2, id);
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-      if (include[c]) part += (uint32_t)(cnt_v2[c] - 1) << 16;
     const uint32_t packed = wave_sum_dpp(part);
-    const int bad_u2 = (int)((packed & 0xFFFFu) >> 1), bad_v2 = (int)((packed >> 16) >> 1);
+    const int bad_u2 = (int)(packed & 0xFFFFu), bad_v2 = (int)(packed >> 16);
     const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
     const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
     const int d = delta < -L.k64 ? -L.k64 : (delta > L.k64 - 1 ? L.k64 - 1 : delta);
@@ -238,19 +258,23 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     const uint64_t s1sq = (uint64_t)s1 * s1;
     const uint64_t big_n = s1sq + mm4 * wu * wv;  // < 2^48 (w <= 2^15, m <= 200)
     // badness of every slot in the moved state (needed for the reverse move)
+    int cnt_u2[CH], cnt_v2[CH], cnt_u[CH], cnt_v[CH], id[CH];
+    bool include[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
+    direction_counts_known<CH>(hist, trash, idu2, include, cnt_u2);
     direction_counts_known<CH>(hist, trash, idu, include, cnt_u);
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+    direction_counts_known<CH>(hist, trash, idv2, include, cnt_v2);
     direction_counts<CH>(table, hist, trash, offset, key_v, key, include, cnt_v, id);
     int nb[CH], w2[CH];
     uint32_t part2 = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      const int bit_old = (int)((on_uv[c] >> lane) & 1ull);
-      const int bit_new = (int)((on_u2v2[c] >> lane) & 1ull);
+      const int bit_old = idu[c] == id_uv ? 1 : 0;
+      const int bit_new = idu2[c] == id_u2v2 ? 1 : 0;
       nb[c] = bad[c];
       if (others[c]) nb[c] += (cnt_u2[c] + cnt_v2[c] - bit_new) - (cnt_u[c] + cnt_v[c] - bit_old);
       if (k == u) nb[c] = bad_u2;
@@ -278,6 +302,9 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
       w[c] = w2[c];
       if (k == u) key[c] = key_u2;
       if (k == v) key[c] = key_v2;
+      // u now holds column cv, v column cu: both lanes of each row
+      if ((k >> 1) == ru) row_word[c] = (int)(((uint32_t)row_word[c] & ~(0xFFu << (8 * (u & 1)))) | ((uint32_t)cv << (8 * (u & 1))));
+      if ((k >> 1) == rv) row_word[c] = (int)(((uint32_t)row_word[c] & ~(0xFFu << (8 * (v & 1)))) | ((uint32_t)cu << (8 * (v & 1))));
     }
     s1 = s1_new;
     build_cdf();

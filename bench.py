@@ -56,15 +56,25 @@ def determinant_ops_per_state(n: int) -> int:
     return 6 * (m * (m - 1) * (m - 2) // 6) + 2 * (m * (m - 1) // 2)
 
 
+def _profile_values() -> dict:
+    """`name: value` lines of the committed rocprofv3 PMC summary of the
+    scoring kernel (HBM bytes per launch and counter-derived fractions)."""
+    path = ROOT / "profiles" / "score_onesided_n75_P65536_pmc.txt"
+    out = {}
+    if path.exists():
+        for line in path.read_text().splitlines():
+            key, sep, value = line.partition(":")
+            if sep and key and " " not in key:
+                try:
+                    out[key] = float(value)
+                except ValueError:
+                    pass
+    return out
+
+
 def traffic_from_profile() -> float | None:
     """HBM bytes per scoring launch from the committed rocprofv3 PMC summary."""
-    path = ROOT / "profiles" / "score_onesided_n75_P65536_pmc.txt"
-    if not path.exists():
-        return None
-    for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+    return _profile_values().get("traffic_bytes_per_launch")
 
 
 def cpu_baseline(seconds: float = 12.0) -> dict:
@@ -333,6 +343,11 @@ def main() -> int:
         lds = POPULATION * slope_lds_accesses_per_state(N) / (kernel_ms * 1e-3) / 1e12
         roofline["lds"] = {"achieved": lds, "peak": PEAK_LDS_TACCESS, "unit": "T accesses/s",
                            "frac": lds / PEAK_LDS_TACCESS, "accesses_per_state": slope_lds_accesses_per_state(N)}
+        # hardware counters of the same kernel (SQ_INSTS_VALU, SQ_LDS_*), as
+        # fractions of the launch's cycles per CU
+        prof = _profile_values()
+        roofline["counters"] = {k: prof.get(k) for k in ("valu_issue_fraction", "lds_active_fraction",
+                                                          "lds_bank_conflict_fraction")}
 
     extra = {}
     legs = set() if args.no_temper else {x.strip() for x in args.legs.split(",") if x.strip()}

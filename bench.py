@@ -251,6 +251,114 @@ def search_rates(pkg) -> dict:
     }
 
 
+MOVE_ANCHORS = (0, 37, 74, 111, 149)
+MOVE_REPEATS = 100
+MOVE_TABLE_STATES = 4096
+
+
+def _move_state(pkg):
+    import numpy as np
+
+    from random import Random
+
+    state = pkg.random_state(N, Random(91001))
+    p = np.asarray(state.P, dtype=np.int32)
+    q = np.asarray(state.Q, dtype=np.int32)
+    score, bad = pkg.score_and_badness(p, q)
+    return p, q, int(score), np.asarray(bad)
+
+
+def move_rates(pkg, torch) -> dict:
+    """Move scores at n=75.
+    - The reference's delta harness (gpu_delta.py:131-157): one seeded state,
+      5 anchors u, 100 repeats of the host entry point score_two_switches_gpu
+      (copies included), and of the same call on device tensors.
+    - The full 2-switch table of 4,096 resident states (150 x 150 scores each).
+    - 3-cycle scores of every legal candidate of one anchor."""
+    import numpy as np
+
+    n, m = N, 2 * N
+    dev = pkg._lib.device(0)
+    lib = pkg._lib.lib()
+    stream = pkg._lib.stream_handle(dev)
+    p, q, score, bad = _move_state(pkg)
+    anchors, repeats = MOVE_ANCHORS, MOVE_REPEATS
+    out = {"n": n, "anchors": len(anchors), "repeats": repeats}
+    for u in anchors:
+        pkg.score_two_switches_gpu(p, q, u, score, bad)  # warm-up
+    t0 = time.perf_counter()
+    for u in anchors:
+        for _ in range(repeats):
+            pkg.score_two_switches_gpu(p, q, u, score, bad)
+    out["host_call_us"] = (time.perf_counter() - t0) / (len(anchors) * repeats) * 1e6
+    tp, tq = torch.from_numpy(p).to(dev), torch.from_numpy(q).to(dev)
+    tb = torch.from_numpy(np.asarray(bad, dtype=np.int32)).to(dev)
+    res = torch.empty(m, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for u in anchors:
+        for _ in range(repeats):
+            pkg._lib.check(lib.n3_two_switch_scores(tp.data_ptr(), tq.data_ptr(), tb.data_ptr(), n, u, int(score),
+                                                    res.data_ptr(), stream))
+    torch.cuda.synchronize()
+    out["device_call_us"] = (time.perf_counter() - t0) / (len(anchors) * repeats) * 1e6
+    # full tables of a resident population
+    count = MOVE_TABLE_STATES
+    pp, pq = pkg.generate_states(5, n, count, device=dev)
+    pbad = torch.empty((count, m), dtype=torch.int32, device=dev)
+    pscores = pkg.score_population_device(pp, pq, badness=pbad)
+    table = torch.empty((count, m, m), dtype=torch.int32, device=dev)
+    pkg.delta_table_device(pp, pq, pscores, pbad, out=table)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pkg.delta_table_device(pp, pq, pscores, pbad, out=table)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    out["table"] = {"states": count, "scores_per_state": m * m, "seconds": el,
+                    "move_scores_per_s": count * m * m / el}
+    # 3-cycles of one anchor
+    cand = pkg.legal_three_cycle_candidates(p, q, 0)
+    pkg.evaluate_three_cycles(p, q, cand, score, bad)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        pkg.evaluate_three_cycles(p, q, cand, score, bad)
+    el = (time.perf_counter() - t0) / 10
+    out["three_cycles"] = {"candidates": int(len(cand)), "seconds_per_call": el,
+                           "candidates_per_s": len(cand) / el}
+    return out
+
+
+def move_cpu_baseline(pkg, oracle) -> dict:
+    """CPU side of the delta harness: the oracle's C loop of
+    accelerated.evaluate_two_switches (one thread) on the same state and
+    anchors, timed per call, and exactness checks of the GPU outputs (the
+    single-state scores and a sample of the resident tables)."""
+    import numpy as np
+
+    p, q, score, bad = _move_state(pkg)
+    bad64 = np.asarray(bad, dtype=np.int64)
+    exact = True
+    for u in MOVE_ANCHORS:
+        ref = oracle.evaluate_two_switches(p, q, u, score, bad64)
+        got = pkg.score_two_switches_gpu(p, q, u, score, bad)
+        legal = ref < (1 << 59)
+        exact &= bool(np.array_equal(got >= (1 << 29), ~legal) and np.array_equal(got[legal], ref[legal]))
+    t0 = time.perf_counter()
+    for u in MOVE_ANCHORS:
+        for _ in range(MOVE_REPEATS):
+            oracle.evaluate_two_switches(p, q, u, score, bad64)
+    call_us = (time.perf_counter() - t0) / (len(MOVE_ANCHORS) * MOVE_REPEATS) * 1e6
+    idx = [0, MOVE_TABLE_STATES // 2, MOVE_TABLE_STATES - 1]
+    hp, hq = oracle.generate_states(5, N, MOVE_TABLE_STATES)
+    table = pkg.delta_table(hp[idx], hq[idx])
+    table_exact = bool(np.array_equal(table, oracle.delta_table(hp[idx], hq[idx])))
+    return {"call_us": call_us, "cores": 1, "kind": "port",
+            "sample": "oracle C loop of accelerated.evaluate_two_switches, n=75, 5 anchors x 100 repeats",
+            "two_switch_exact": exact, "table_sample_exact": table_exact}
+
+
 def calibrate_copy() -> int:
     """A plain device-to-device copy of 256 MiB, five times: profiled with
     FETCH_SIZE / WRITE_SIZE it gives the counters' reading for known bytes."""
@@ -274,8 +382,8 @@ def main() -> int:
     ap.add_argument("--no-temper", action="store_true", help="skip the tempering, batched and learned legs")
     ap.add_argument("--temper-sweeps", type=int, default=10000,
                     help="sweeps of the tempering leg (config 3 is 10,000; fewer for counter profiles)")
-    ap.add_argument("--legs", default="temper,batched,learned,near,search",
-                    help="comma list of the extra legs to run: temper, batched, learned, near, search")
+    ap.add_argument("--legs", default="temper,batched,learned,near,search,moves",
+                    help="comma list of the extra legs to run: temper, batched, learned, near, search, moves")
     ap.add_argument("--ab-seconds", type=float, default=20.0, help="wall-clock budget of each A/B side")
     ap.add_argument("--near-seconds", type=float, default=4.0,
                     help="wall-clock budget of each side of the near-solution A/B")
@@ -320,17 +428,6 @@ def main() -> int:
     value = POPULATION * args.steps / wall
     log(f"bench: {args.steps} steps, {ms_per_step:.3f} ms/step wall, {kernel_ms:.3f} ms/launch (events)")
 
-    # parity spot check of this run's outputs on a subsample
-    import numpy as np
-
-    oracle = entry.load_oracle()
-    idx = np.arange(0, POPULATION, POPULATION // 64)
-    hp, hq = p[idx].cpu().numpy(), q[idx].cpu().numpy()
-    ref_s, ref_b = oracle.score_population(hp, hq)
-    exact = bool(np.array_equal(scores[idx].cpu().numpy(), ref_s) and np.array_equal(bad[idx].cpu().numpy(), ref_b))
-    if not exact:
-        log("bench: PARITY FAILURE on the subsample")
-
     ops = slope_ops_per_state(N) if args.method != 1 else determinant_ops_per_state(N)
     achieved = POPULATION * ops / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "TOP/s",
@@ -351,7 +448,7 @@ def main() -> int:
 
     extra = {}
     legs = set() if args.no_temper else {x.strip() for x in args.legs.split(",") if x.strip()}
-    unknown = legs - {"temper", "batched", "learned", "near", "search"}
+    unknown = legs - {"temper", "batched", "learned", "near", "search", "moves"}
     if unknown:
         log(f"bench: unknown legs {sorted(unknown)}")
         return 2
@@ -372,11 +469,34 @@ def main() -> int:
     if "search" in legs:
         log("bench: device GA and exhaustive beam")
         extra["search_engines"] = search_rates(pkg)
+    if "moves" in legs:
+        log("bench: move scores")
+        extra["move_scores"] = move_rates(pkg, torch)
     cpu = None
+    exact = None
     if not args.no_cpu:
+        # the CPU leg: the oracle as the baseline and as the parity checker
+        # of this run's outputs
+        import numpy as np
+
+        oracle = entry.load_oracle()
+        idx = np.arange(0, POPULATION, POPULATION // 64)
+        hp, hq = p[idx].cpu().numpy(), q[idx].cpu().numpy()
+        ref_s, ref_b = oracle.score_population(hp, hq)
+        exact = bool(np.array_equal(scores[idx].cpu().numpy(), ref_s) and
+                     np.array_equal(bad[idx].cpu().numpy(), ref_b))
+        if not exact:
+            log("bench: PARITY FAILURE on the subsample")
         log("bench: CPU baseline")
         cpu = cpu_baseline()
         cpu["gpu_over_cpu"] = value / cpu["value"]
+        if "move_scores" in extra:
+            moves_cpu = move_cpu_baseline(pkg, oracle)
+            extra["move_scores"]["cpu_baseline"] = moves_cpu
+            extra["move_scores"]["host_call_speedup"] = moves_cpu["call_us"] / extra["move_scores"]["host_call_us"]
+            if not (moves_cpu["two_switch_exact"] and moves_cpu["table_sample_exact"]):
+                log("bench: PARITY FAILURE in the move scores")
+                exact = False
 
     line = {
         "metric": "n=75 states scored/s + time-to-E≤12 (learned vs uniform proposals), 1 GPU",
@@ -394,7 +514,7 @@ def main() -> int:
         **extra,
     }
     print(json.dumps(line), flush=True)
-    return 0 if exact else 1
+    return 0 if exact is not False else 1
 
 
 if __name__ == "__main__":

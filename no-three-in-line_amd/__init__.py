@@ -11,6 +11,7 @@ from ._lib import SCORE_AUTO, SCORE_DETERMINANT, SCORE_SLOPE, SENTINEL32, SENTIN
 from .gpu import (
     check_status,
     generate_states,
+    score_and_badness,
     score_population_badness,
     score_population_device,
     score_population_gpu,
@@ -36,7 +37,7 @@ from .gpu_exhaustive_beam import candidate_descriptors, gpu_exhaustive_beam_sear
 
 __all__ = [
     "SCORE_AUTO", "SCORE_DETERMINANT", "SCORE_SLOPE", "SENTINEL32", "SENTINEL64",
-    "check_status", "generate_states", "score_population_badness", "score_population_device", "score_population_gpu",
+    "check_status", "generate_states", "score_and_badness", "score_population_badness", "score_population_device", "score_population_gpu",
     "delta_table", "delta_table_device", "evaluate_three_cycles", "evaluate_transpositions",
     "apply_three_cycle", "legal_three_cycle_candidates", "evaluate_two_switches", "score_two_switches_gpu",
     "PairPermutations", "points_to_permutations", "random_state", "random_two_switches", "TemperingChains", "_lib",

@@ -290,13 +290,9 @@ static int launch_switch(const SwitchArgs& args, hipStream_t stream) {
   DirTableInfo info{};
   int st = device_dirtable(args.n, &table, &info);
   if (st) return st;
-  hipDeviceProp_t prop;
-  int device = 0;
-  (void)hipGetDevice(&device);
-  st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  int lds_limit = 0, compute_units = 0;
+  st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
-  int lds_limit = (int)prop.sharedMemPerBlock;
-  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
   const int table_bytes = table_lds_bytes(info), hist_bytes = info.hist_words * 4;
   int waves = (lds_limit - table_bytes) / hist_bytes;
   if (waves > 8) waves = 8;
@@ -304,7 +300,7 @@ static int launch_switch(const SwitchArgs& args, hipStream_t stream) {
   const long long anchors = args.u_fixed >= 0 ? 1 : 2 * args.n;
   const long long items = args.states * anchors * args.splits;
   long long blocks = (items + waves - 1) / waves;
-  const long long max_blocks = (long long)prop.multiProcessorCount * 2;
+  const long long max_blocks = (long long)compute_units * 2;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
   const size_t lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
@@ -372,7 +368,8 @@ extern "C" int n3_two_switch_scores(const int32_t* p, const int32_t* q, const in
   int st = check_switch(n, p, q, badness, scores);
   if (st) return st;
   if (u < 0 || u >= 2 * n) return set_error(N3_EINVAL, "selected index %d outside [0, %d)", u, 2 * n);
-  SwitchArgs a{p, q, n, 1, u, -1, 8, badness, nullptr, current_score, scores, 2 * n, nullptr};
+  // one wavefront per partner: a single state's call is latency-bound
+  SwitchArgs a{p, q, n, 1, u, -1, 2 * n, badness, nullptr, current_score, scores, 2 * n, nullptr};
   return run_switch(a, false, as_stream(stream));
 }
 
@@ -381,7 +378,7 @@ extern "C" int n3_evaluate_two_switches(const int32_t* p, const int32_t* q, cons
   int st = check_switch(n, p, q, badness, scores);
   if (st) return st;
   if (u < 0 || u >= 2 * n) return set_error(N3_EINVAL, "selected index %d outside [0, %d)", u, 2 * n);
-  SwitchArgs a{p, q, n, 1, u, -1, 8, badness, nullptr, (long long)current_score, scores, 2 * n, nullptr};
+  SwitchArgs a{p, q, n, 1, u, -1, 2 * n, badness, nullptr, (long long)current_score, scores, 2 * n, nullptr};
   return run_switch(a, true, as_stream(stream));
 }
 
@@ -391,7 +388,7 @@ extern "C" int n3_evaluate_transpositions(const int32_t* p, const int32_t* q, co
   if (st) return st;
   if (layer != 0 && layer != 1) return set_error(N3_EINVAL, "layer must be 0 (P) or 1 (Q)");
   if (row < 0 || row >= n) return set_error(N3_EINVAL, "row %d outside [0, %d)", row, n);
-  SwitchArgs a{p, q, n, 1, 2 * row + layer, layer, 8, badness, nullptr, (long long)current_score, scores, n, nullptr};
+  SwitchArgs a{p, q, n, 1, 2 * row + layer, layer, n, badness, nullptr, (long long)current_score, scores, n, nullptr};
   return run_switch(a, true, as_stream(stream));
 }
 

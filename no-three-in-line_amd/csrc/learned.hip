@@ -369,13 +369,9 @@ struct LearnedPlan {
 static int learned_plan(const n3_chains* c, LearnedPlan* plan) {
   int st = device_dirtable(c->n, &plan->table, &plan->info);
   if (st) return st;
-  hipDeviceProp_t prop;
-  int device = 0;
-  (void)hipGetDevice(&device);
-  st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  int lds_limit = 0, compute_units = 0;
+  st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
-  int lds_limit = (int)prop.sharedMemPerBlock;
-  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
   const int table_bytes = table_lds_bytes(plan->info), hist_bytes = plan->info.hist_words * 4;
   const int fit = (lds_limit - table_bytes) / hist_bytes;
   if (fit < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", c->n);
@@ -386,7 +382,7 @@ static int learned_plan(const n3_chains* c, LearnedPlan* plan) {
                        c->n);
     waves = c->block_waves;
   } else {
-    const int per_cu = (c->replicas + prop.multiProcessorCount - 1) / prop.multiProcessorCount;
+    const int per_cu = (c->replicas + compute_units - 1) / compute_units;
     if (waves > per_cu) waves = per_cu < 1 ? 1 : per_cu;
   }
   plan->waves = waves;

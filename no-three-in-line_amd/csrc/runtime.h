@@ -14,6 +14,9 @@ int build_dirtable(int n, std::vector<uint16_t>& table, DirTableInfo& info);
 // Resident direction table of size n on the current device (built on first use).
 int device_dirtable(int n, const uint16_t** table, DirTableInfo* info);
 int check_columns_launch_args(int n, int64_t count);
+// LDS bytes per block (at most 160 KiB) and compute-unit count of the current
+// device, queried once per device.
+int device_limits(int* lds_limit, int* compute_units);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -2,6 +2,7 @@
 // per-(device, n) direction tables kept resident in HBM.
 #include "runtime.h"
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -117,6 +118,34 @@ int device_dirtable(int n, const uint16_t** table, DirTableInfo* info) {
   g_tables.push_back(t);
   *table = t.data;
   *info = t.info;
+  return N3_OK;
+}
+
+int device_limits(int* lds_limit, int* compute_units) {
+  struct Limits {
+    std::atomic<int> ready{0};
+    int lds = 0, cus = 0;
+  };
+  static Limits limits[64];
+  int device = 0;
+  int st = hip_status(hipGetDevice(&device), "hipGetDevice");
+  if (st) return st;
+  if (device < 0 || device >= 64) return set_error(N3_ENODEV, "device index %d out of range", device);
+  Limits& l = limits[device];
+  if (!l.ready.load(std::memory_order_acquire)) {
+    int lds = 0, cus = 0;
+    st = hip_status(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device),
+                    "hipDeviceGetAttribute(shared memory)");
+    if (st) return st;
+    st = hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
+                    "hipDeviceGetAttribute(compute units)");
+    if (st) return st;
+    l.lds = lds > 160 * 1024 ? 160 * 1024 : lds;
+    l.cus = cus;
+    l.ready.store(1, std::memory_order_release);
+  }
+  *lds_limit = l.lds;
+  *compute_units = l.cus;
   return N3_OK;
 }
 

@@ -613,13 +613,9 @@ static int wide_table(int n, const DirTableInfo& info, OneSidedTable* out) {
 template <int CH>
 static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
                            const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
-  hipDeviceProp_t prop;
-  int device = 0;
-  (void)hipGetDevice(&device);
-  int st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  int lds_limit = 0, compute_units = 0;
+  int st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
-  int lds_limit = (int)prop.sharedMemPerBlock;
-  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
   const int hist_bytes = info.hist_words * 4;
   const int wide_bytes = (wide_stride(n) * info.width * 4 + 15) / 16 * 16;
   OneSidedTable tab{table, (n * info.width * 2 + 15) / 16 * 16, info.width, 1};
@@ -641,7 +637,7 @@ static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t co
     lds = (size_t)tab.bytes + (size_t)waves * hist_bytes;
   }
   long long blocks = ((long long)count + waves - 1) / waves;
-  const long long max_blocks = (long long)prop.multiProcessorCount;
+  const long long max_blocks = (long long)compute_units;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
   const dim3 grid((unsigned)blocks), block(waves * kWave);
@@ -660,13 +656,9 @@ static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t co
 template <int CH>
 static int launch_slope(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
                         const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
-  hipDeviceProp_t prop;
-  int device = 0;
-  (void)hipGetDevice(&device);
-  int st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  int lds_limit = 0, compute_units = 0;
+  int st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
-  int lds_limit = (int)prop.sharedMemPerBlock;
-  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
   const int table_bytes = table_lds_bytes(info);
   const int hist_bytes = info.hist_words * 4;
   int waves = (lds_limit - table_bytes) / hist_bytes;
@@ -675,7 +667,7 @@ static int launch_slope(const int32_t* p, const int32_t* q, int n, int64_t count
   const size_t lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
   // the LDS budget admits one block per CU; the blocks stride over the states
   long long blocks = ((long long)count + waves - 1) / waves;
-  const long long max_blocks = (long long)prop.multiProcessorCount;
+  const long long max_blocks = (long long)compute_units;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(score_slope_kernel<CH>, dim3((unsigned)blocks), dim3(waves * kWave), lds, stream, p, q, n,

@@ -523,13 +523,9 @@ struct WalkPlan {
 static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan) {
   int st = device_dirtable(n, &plan->table, &plan->info);
   if (st) return st;
-  hipDeviceProp_t prop;
-  int device = 0;
-  (void)hipGetDevice(&device);
-  st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  int lds_limit = 0, compute_units = 0;
+  st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
-  int lds_limit = (int)prop.sharedMemPerBlock;
-  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
   const int table_bytes = table_lds_bytes(plan->info), hist_bytes = plan->info.hist_words * 4;
   const int fit = (lds_limit - table_bytes) / hist_bytes;
   if (fit < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
@@ -540,7 +536,7 @@ static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan) {
     waves = block_waves;
   } else {
     // spread small populations over more CUs
-    const int per_cu = (replicas + prop.multiProcessorCount - 1) / prop.multiProcessorCount;
+    const int per_cu = (replicas + compute_units - 1) / compute_units;
     if (waves > per_cu) waves = per_cu < 1 ? 1 : per_cu;
   }
   plan->waves = waves;
@@ -672,13 +668,9 @@ struct BatchPlan {
 };
 
 static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
-  hipDeviceProp_t prop;
-  int device = 0;
-  (void)hipGetDevice(&device);
-  int st = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  int lds_limit = 0, compute_units = 0;
+  int st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
-  int lds_limit = (int)prop.sharedMemPerBlock;
-  if (lds_limit > 160 * 1024) lds_limit = 160 * 1024;
   std::memset(&plan->host, 0, sizeof(BatchTables));
   // largest sizes first: their blocks run longest, so they are dispatched first
   int order[N3_MAX_GROUPS];

@@ -115,38 +115,53 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
       const int key_u2 = ru * width + cv;  // u' = (row_u, col_v)
       const int key_v2 = rv * width + cu;  // v' = (row_v, col_u)
       const int id_u2v2 = table[key_v2 - key_u2 + offset];
-      int keys[CH], cnt[CH], id[CH];
-      bool include[CH];
-      // histogram seen from u' over the moved state without u'
+      // Histograms seen from u' and v' over the moved state: bad'(x) is the
+      // sum of the ranks the increments return (sum_d C(cnt_d, 2)), so both
+      // are built and cleared back to back with one wait.  The moved slot
+      // itself looks up the zero vector (rank 0; the legality rules leave
+      // no point on u' or v'); lanes past the last slot use their trash
+      // word, in the last chunk only.
+      int idu2[CH], idv2[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int k = lane + kWave * c;
-        include[c] = (k < m) && k != u;
-        keys[c] = (k == v) ? key_v2 : key[c];
+        const int moved = (k == v) ? key_v2 : ((k == u) ? key_u2 : key[c]);
+        idu2[c] = table[moved - key_u2 + offset];
+        idv2[c] = table[moved - key_v2 + offset];
       }
-      direction_counts<CH>(table, hist, trash, offset, key_u2, keys, include, cnt, id);
+      const bool tail_in = lane + kWave * (CH - 1) < m;
+      uint32_t word_u2[CH], word_v2[CH], old_u2[CH], old_v2[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
+        old_u2[c] = atomicAdd(&hist[word_u2[c]], 1u << ((uint32_t)idu2[c] << 3));
+      }
+      wave_sync();
+#pragma unroll
+      for (int c = 0; c < CH; ++c) hist[word_u2[c]] = 0u;
+      wave_sync();
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
+        old_v2[c] = atomicAdd(&hist[word_v2[c]], 1u << ((uint32_t)idv2[c] << 3));
+      }
+      wave_sync();
+#pragma unroll
+      for (int c = 0; c < CH; ++c) hist[word_v2[c]] = 0u;
+      wave_sync();
+      // keep the rank extraction below every LDS step above
+      __builtin_amdgcn_sched_barrier(0);
       uint32_t part = 0;
       int joint_new = 0;
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int k = lane + kWave * c;
-        if (include[c]) part += (uint32_t)(cnt[c] - 1);
-        joint_new += __popcll(__ballot(include[c] && k != v && id[c] == id_u2v2));
+        part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
+                __builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8);
+        joint_new += __popcll(__ballot((k < m) && k != u && k != v && idu2[c] == id_u2v2));
       }
-      // histogram seen from v' over the moved state without v'
-#pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int k = lane + kWave * c;
-        include[c] = (k < m) && k != v;
-        keys[c] = (k == u) ? key_u2 : key[c];
-      }
-      direction_counts<CH>(table, hist, trash, offset, key_v2, keys, include, cnt, id);
-#pragma unroll
-      for (int c = 0; c < CH; ++c)
-        if (include[c]) part += (uint32_t)(cnt[c] - 1);
-      // part sums (cnt - 1) over both anchors: twice bad'(u') + bad'(v')
-      const uint32_t twice_bad = wave_sum_dpp(part);
-      const long long new_affected = (long long)(twice_bad >> 1) - joint_new;
+      // bad'(u') + bad'(v') (each below 2^16, so the sum fits one word)
+      const long long new_affected = (long long)wave_sum_dpp(part) - joint_new;
       const long long bad_v = load_value<BT>(a.badness, s * m + v);
       const long long value = current - (bad_u + bad_v - joint_old) + new_affected;
       if (lane == 0) out[t] = (OT)value;

@@ -254,47 +254,57 @@ __global__ void __launch_bounds__(256) three_cycle_kernel(const int32_t* __restr
     }
     const int ku = ru * width + nu, kv = rv * width + nv, kw = rw * width + nw;
     const int id2_uv = table[kv - ku + offset], id2_uw = table[kw - ku + offset], id2_vw = table[kw - kv + offset];
-    int keys[CH], cnt[CH], id[CH];
-    bool include[CH];
+    // Histograms seen from u', v' and w' over the moved state (each without
+    // its own anchor), built and cleared back to back: bad'(x) is the sum of
+    // the ranks the increments return.  Excluded lanes update their trash
+    // word, which a lane shares across its chunks, and add no rank.
+    int idu2[CH], idv2[CH], idw2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      keys[c] = (k == u) ? ku : ((k == v) ? kv : ((k == w) ? kw : key[c]));
+      const int moved = (k == u) ? ku : ((k == v) ? kv : ((k == w) ? kw : key[c]));
+      idu2[c] = table[moved - ku + offset];
+      idv2[c] = table[moved - kv + offset];
+      idw2[c] = table[moved - kw + offset];
     }
+    uint32_t rank_u[CH], rank_v[CH], rank_w[CH];
+    auto build = [&](const int(&id)[CH], int anchor_slot, uint32_t(&old)[CH]) {
+      uint32_t word[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int k = lane + kWave * c;
+        word[c] = (k < m && k != anchor_slot) ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
+        old[c] = atomicAdd(&hist[word[c]], 1u << ((uint32_t)id[c] << 3));
+      }
+      wave_sync();
+#pragma unroll
+      for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
+      wave_sync();
+    };
+    build(idu2, u, rank_u);
+    build(idv2, v, rank_v);
+    build(idw2, w, rank_w);
+    // keep the rank extraction below every LDS step above
+    __builtin_amdgcn_sched_barrier(0);
     uint32_t part_uv = 0, part_w = 0;
     int joint_new = 0;
-    // from u'
-#pragma unroll
-    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
-    direction_counts<CH>(table, hist, trash, offset, ku, keys, include, cnt, id);
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int k = lane + kWave * c;
-      if (include[c]) part_uv += (uint32_t)(cnt[c] - 1);
-      joint_new += __popcll(__ballot(include[c] && k != v && id[c] == id2_uv));
-      joint_new += __popcll(__ballot(include[c] && k != w && id[c] == id2_uw));
-    }
-    // from v'
-#pragma unroll
-    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
-    direction_counts<CH>(table, hist, trash, offset, kv, keys, include, cnt, id);
+    auto rank_of = [&](uint32_t old, int id, int k, int anchor_slot) -> uint32_t {
+      return (k < m && k != anchor_slot) ? __builtin_amdgcn_ubfe(old, (uint32_t)id << 3, 8) : 0u;
+    };
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      if (include[c]) part_uv += (uint32_t)(cnt[c] - 1) << 16;
-      joint_new += __popcll(__ballot(include[c] && k != w && id[c] == id2_vw));
+      const bool live = k < m;
+      part_uv += rank_of(rank_u[c], idu2[c], k, u) + (rank_of(rank_v[c], idv2[c], k, v) << 16);
+      part_w += rank_of(rank_w[c], idw2[c], k, w);
+      joint_new += __popcll(__ballot(live && k != u && k != v && idu2[c] == id2_uv));
+      joint_new += __popcll(__ballot(live && k != u && k != w && idu2[c] == id2_uw));
+      joint_new += __popcll(__ballot(live && k != v && k != w && idv2[c] == id2_vw));
     }
-    // from w'
-#pragma unroll
-    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != w;
-    direction_counts<CH>(table, hist, trash, offset, kw, keys, include, cnt, id);
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-      if (include[c]) part_w += (uint32_t)(cnt[c] - 1);
     const uint32_t packed = wave_sum_dpp(part_uv), total_w = wave_sum_dpp(part_w);
     const int triple_new = (id2_uv == id2_uw) ? 1 : 0;
-    const long long new_affected = (long long)((packed & 0xFFFFu) >> 1) + (long long)((packed >> 16) >> 1) +
-                                   (long long)(total_w >> 1) - joint_new + triple_new;
+    const long long new_affected = (long long)(packed & 0xFFFFu) + (long long)(packed >> 16) + (long long)total_w -
+                                   joint_new + triple_new;
     if (lane == 0) out[ci] = current - old_affected + new_affected;
   }
 }

@@ -342,7 +342,12 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
 //             stride S = 16 (mod 32) words: the points of 16 consecutive rows
 //             that a half-wave looks up fall in distinct bank pairs, where
 //             the row-major ids fall in random banks.
-enum OneSidedLayout { kHalf16 = 0, kPacked16 = 1, kWide32 = 2 };
+//  kNibble16: 2-byte entries 4 id (row-major half table) and 4-bit counters,
+//             8 per word: half the histogram, so two 16-wave blocks share a CU.
+//             A counter that would pass 15 (16 earlier points on one line
+//             through the anchor) is detected from the returned old value
+//             and the state is recounted by the determinant path.
+enum OneSidedLayout { kHalf16 = 0, kPacked16 = 1, kWide32 = 2, kNibble16 = 3 };
 constexpr int kPackedTableBytes = 32 * 1024;
 constexpr int kPackedHistBytes = 8 * 1024;
 constexpr int kPackedWaves = 16;
@@ -352,6 +357,7 @@ struct OneSidedTable {
   int bytes;          // staged bytes (multiple of 16)
   int row_stride;     // entries between rows
   int col_stride;     // entries between columns
+  int hist_words;     // 32-bit words of one wave's histogram
 };
 
 // After anchor a's histogram of the directions to the slots b < a is built,
@@ -380,18 +386,19 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
   } else {
     extern __shared__ __align__(16) unsigned char smem[];
     lds = smem;
-    hist_base = (uint32_t)(tab.bytes + wave * info.hist_words * 4);
+    hist_base = (uint32_t)(tab.bytes + wave * tab.hist_words * 4);
   }
   uint32_t* hist = reinterpret_cast<uint32_t*>(lds + hist_base);
   // counter word of an entry and the counter's bit position in it
   auto counter = [&](uint32_t e) -> uint32_t* {
     if constexpr (kLayout == kWide32) return reinterpret_cast<uint32_t*>(lds + hist_base + (e >> 16));
+    if constexpr (kLayout == kNibble16) return reinterpret_cast<uint32_t*>(lds + hist_base + ((e >> 3) & ~3u));
     if constexpr (kLayout == kPacked16) return reinterpret_cast<uint32_t*>(lds + (hist_base | (e & ~3u)));
     return reinterpret_cast<uint32_t*>(lds + hist_base + (e & ~3u));
   };
   // the low five bits are the shift; v_lshlrev and v_bfe read only those
   auto shift = [](uint32_t e) -> uint32_t {
-    if constexpr (kLayout == kWide32) return e;
+    if constexpr (kLayout == kWide32 || kLayout == kNibble16) return e;
     return e << 3;
   };
   {
@@ -399,8 +406,9 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
     uint2* dst = reinterpret_cast<uint2*>(lds);
     for (int i = threadIdx.x; i < tab.bytes / 8; i += blockDim.x) dst[i] = src[i];
   }
-  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+  for (int i = lane; i < tab.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
+  constexpr int counter_bits = (kLayout == kNibble16) ? 4 : 8;
 
   const int m = 2 * n;
   const int width = info.width;
@@ -487,10 +495,21 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
         e[c] = *reinterpret_cast<const Entry*>(entry);
       }
     };
+    // 4-bit counters: a lane whose increment finds 15 would carry into the
+    // next counter; it flags the state for the determinant recount
+    bool overflow = false;
     auto increment = [&](int ca, int l, const uint32_t(&e)[CH]) {
 #pragma unroll
-      for (int c = 0; c <= ca; ++c)
-        if (c < ca || lane < l) atomicAdd(counter(e[c]), 1u << (shift(e[c]) & 31u));
+      for (int c = 0; c <= ca; ++c) {
+        if (c < ca || lane < l) {
+          if constexpr (kLayout == kNibble16) {
+            const uint32_t old = atomicAdd(counter(e[c]), 1u << (shift(e[c]) & 31u));
+            overflow |= __builtin_amdgcn_ubfe(old, shift(e[c]), 4) == 15u;
+          } else {
+            atomicAdd(counter(e[c]), 1u << (shift(e[c]) & 31u));
+          }
+        }
+      }
       wave_sync();
     };
     auto collect = [&](int ca, const uint32_t(&e)[CH], uint32_t(&word)[CH]) {
@@ -504,7 +523,7 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
       uint32_t part = 0;
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
-        const uint32_t cnt = __builtin_amdgcn_ubfe(word[c], shift(e[c]), 8);
+        const uint32_t cnt = __builtin_amdgcn_ubfe(word[c], shift(e[c]), counter_bits);
         counts[c] += (int)cnt;
         part += cnt;
       }
@@ -543,6 +562,36 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
         own[ca] = (lane == l + 1) ? second_own : own[ca];
       }
     }
+    if constexpr (kLayout == kNibble16) {
+      if (__any(overflow)) {
+        // a line through some anchor holds 16 or more earlier points: the
+        // exact determinant count, with the (cleared) histogram as scratch
+        int32_t* cols = reinterpret_cast<int32_t*>(hist);
+        int32_t* credit = cols + m;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const int k = lane + kWave * c;
+          if (k < m) cols[k] = key[c] - (k >> 1) * width;
+        }
+        wave_sync();
+        int bad[CH];
+        determinant_state<CH>(cols, credit, m, bad);
+        int sum = 0;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const int k = lane + kWave * c;
+          if (k < m) {
+            sum += bad[c];
+            if (badness) badness[s * m + k] = bad[c];
+          }
+        }
+        sum = wave_sum(sum);
+        if (lane == 0) scores[s] = sum / 3;
+        for (int i = lane; i < 2 * m; i += kWave) hist[i] = 0;
+        wave_sync();
+        continue;
+      }
+    }
     // slot k was an earlier slot of the anchors k+1 .. m-1, and each of them
     // adds its count, which includes k itself
     int sum = 0;
@@ -563,53 +612,76 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
 // Column stride of the wide layout: at least n, and 16 (mod 32).
 inline int wide_stride(int n) { return ((n + 15) / 32) * 32 + 16; }
 
-// Resident wide-layout tables (built from the direction ids on first use).
-static int wide_table(int n, const DirTableInfo& info, OneSidedTable* out) {
-  struct Wide {
-    int device, n;
-    uint32_t* data;
-    int bytes;
+// Words of one wave's 4-bit-counter histogram: the D + 1 counters, at least
+// 4n words (the determinant recount's scratch), a multiple of 16 words.
+inline int nibble_hist_words(const DirTableInfo& info) {
+  int words = (info.directions + 1 + 7) / 8;
+  if (words < 4 * info.n) words = 4 * info.n;
+  return (words + 15) / 16 * 16;
+}
+
+// Resident one-sided tables per (device, n, layout), built from the direction
+// ids on first use: kWide32 (4-byte entries, dr fastest, column stride S)
+// and kNibble16 (2-byte entries 4 id, row-major).
+static int onesided_table(int n, int layout, const DirTableInfo& info, OneSidedTable* out) {
+  struct Resident {
+    int device, n, layout;
+    OneSidedTable tab;
   };
   static std::mutex mutex;
-  static std::vector<Wide> tables;
+  static std::vector<Resident> tables;
   int device = 0;
   int st = hip_status(hipGetDevice(&device), "hipGetDevice");
   if (st) return st;
-  const int stride = wide_stride(n), w = info.width, span = n - 1;
   std::lock_guard<std::mutex> lock(mutex);
   for (const auto& t : tables) {
-    if (t.device == device && t.n == n) {
-      *out = OneSidedTable{t.data, t.bytes, 1, stride};
+    if (t.device == device && t.n == n && t.layout == layout) {
+      *out = t.tab;
       return N3_OK;
     }
   }
-  std::vector<uint16_t> host_ids;
+  std::vector<uint16_t> ids;
   DirTableInfo built{};
-  st = build_dirtable(n, host_ids, built);
+  st = build_dirtable(n, ids, built);
   if (st) return st;
-  const int words = stride * w;
-  std::vector<uint32_t> host(((size_t)words + 3) / 4 * 4, 0u);
-  for (int dr = 0; dr <= span; ++dr) {      // rows dr - span = -(n-1) .. 0
-    for (int dc = 0; dc < w; ++dc) {
-      const uint32_t id = host_ids[(size_t)dr * w + dc];
-      host[(size_t)dr + (size_t)stride * dc] = ((id & ~3u) << 16) | ((id & 3u) << 3);
-    }
+  const int w = info.width, span = n - 1;
+  std::vector<uint32_t> host;  // 32-bit words of the table, padded to 16 bytes
+  OneSidedTable tab{};
+  if (layout == kWide32) {
+    const int stride = wide_stride(n);
+    host.assign(((size_t)stride * w + 3) / 4 * 4, 0u);
+    for (int dr = 0; dr <= span; ++dr)  // rows dr - span = -(n-1) .. 0
+      for (int dc = 0; dc < w; ++dc) {
+        const uint32_t id = ids[(size_t)dr * w + dc];
+        host[(size_t)dr + (size_t)stride * dc] = ((id & ~3u) << 16) | ((id & 3u) << 3);
+      }
+    tab = OneSidedTable{nullptr, (int)(host.size() * 4), 1, stride, info.hist_words};
+  } else {
+    const size_t entries = (size_t)n * w;
+    host.assign((entries * 2 + 15) / 16 * 4, 0u);
+    uint16_t* h16 = reinterpret_cast<uint16_t*>(host.data());
+    for (size_t i = 0; i < entries; ++i) h16[i] = (uint16_t)(4u * ids[i]);
+    tab = OneSidedTable{nullptr, (int)(host.size() * 4), w, 1, nibble_hist_words(info)};
   }
-  Wide t{device, n, nullptr, (int)(host.size() * 4)};
-  st = hip_status(hipMalloc(&t.data, host.size() * 4), "hipMalloc(wide table)");
+  void* data = nullptr;
+  st = hip_status(hipMalloc(&data, host.size() * 4), "hipMalloc(one-sided table)");
   if (st) return st;
-  st = hip_status(hipMemcpy(t.data, host.data(), host.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(wide table)");
+  st = hip_status(hipMemcpy(data, host.data(), host.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(one-sided table)");
   if (st) {
-    (void)hipFree(t.data);
+    (void)hipFree(data);
     return st;
   }
-  tables.push_back(t);
-  *out = OneSidedTable{t.data, t.bytes, 1, stride};
+  tab.data = data;
+  tables.push_back(Resident{device, n, layout, tab});
+  *out = tab;
   return N3_OK;
 }
 
 // Layout choice: the wide table where it leaves room for 16 histograms
-// (n <= 75), else the packed layout where it fits (n <= 80), else kHalf16.
+// (n <= 75), else the packed layout where it fits (n <= 80), else kHalf16
+// where 16 byte-counter histograms fit (n <= 83), else kNibble16, whose
+// histograms are half the size.  Two blocks share a CU when their LDS fits
+// (the wide layout at the smaller sizes).
 template <int CH>
 static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
                            const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
@@ -618,26 +690,32 @@ static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t co
   if (st) return st;
   const int hist_bytes = info.hist_words * 4;
   const int wide_bytes = (wide_stride(n) * info.width * 4 + 15) / 16 * 16;
-  OneSidedTable tab{table, (n * info.width * 2 + 15) / 16 * 16, info.width, 1};
+  OneSidedTable tab{table, (n * info.width * 2 + 15) / 16 * 16, info.width, 1, info.hist_words};
   int layout = kHalf16;
   if (wide_bytes + kPackedWaves * hist_bytes <= lds_limit) {
     layout = kWide32;
-    st = wide_table(n, info, &tab);
-    if (st) return st;
   } else if (tab.bytes <= kPackedTableBytes && hist_bytes <= kPackedHistBytes &&
              lds_limit >= kPackedTableBytes + kPackedWaves * kPackedHistBytes) {
     layout = kPacked16;
+  } else if (tab.bytes + 16 * hist_bytes > lds_limit) {
+    layout = kNibble16;
   }
-  int waves = kPackedWaves;
+  if (layout == kWide32 || layout == kNibble16) {
+    st = onesided_table(n, layout, info, &tab);
+    if (st) return st;
+  }
+  int waves = kPackedWaves, per_cu = 1;
   size_t lds = 0;
   if (layout != kPacked16) {
-    waves = (lds_limit - tab.bytes) / hist_bytes;
+    const int bytes = tab.hist_words * 4;
+    waves = (lds_limit - tab.bytes) / bytes;
     if (waves > 16) waves = 16;
     if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
-    lds = (size_t)tab.bytes + (size_t)waves * hist_bytes;
+    lds = (size_t)tab.bytes + (size_t)waves * bytes;
+    if (2 * lds <= (size_t)lds_limit) per_cu = 2;  // two blocks share a CU
   }
   long long blocks = ((long long)count + waves - 1) / waves;
-  const long long max_blocks = (long long)compute_units;
+  const long long max_blocks = (long long)compute_units * per_cu;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
   const dim3 grid((unsigned)blocks), block(waves * kWave);
@@ -646,6 +724,9 @@ static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t co
                        scores, badness, tab, info, status);
   else if (layout == kWide32)
     hipLaunchKernelGGL((score_onesided_kernel<CH, kWide32>), grid, block, lds, stream, p, q, n, (long long)count,
+                       scores, badness, tab, info, status);
+  else if (layout == kNibble16)
+    hipLaunchKernelGGL((score_onesided_kernel<CH, kNibble16>), grid, block, lds, stream, p, q, n, (long long)count,
                        scores, badness, tab, info, status);
   else
     hipLaunchKernelGGL((score_onesided_kernel<CH, kHalf16>), grid, block, lds, stream, p, q, n, (long long)count,

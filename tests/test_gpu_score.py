@@ -32,7 +32,8 @@ def test_generator_matches_oracle(pkg, oracle, dev, n, count, seed):
 @pytest.mark.parametrize("method", [0, 1, 2])
 @pytest.mark.parametrize("n,count,seed", [(2, 8, 1), (3, 16, 9), (7, 64, 2), (32, 256, 0), (33, 64, 3),
                                           (48, 64, 8), (64, 64, 4), (65, 40, 10), (75, 96, 5), (78, 48, 11),
-                                          (80, 32, 12), (83, 32, 7), (97, 16, 13), (100, 24, 6)])
+                                          (80, 32, 12), (83, 32, 7), (84, 24, 14), (90, 16, 15), (97, 16, 13),
+                                          (100, 24, 6)])
 def test_scores_and_badness_match_oracle(pkg, oracle, dev, method, n, count, seed):
     hp, hq = oracle.generate_states(seed, n, count)
     scores, bad = _score(pkg, hp, hq, dev, method)
@@ -90,7 +91,7 @@ def test_coincident_points_and_non_permutations(pkg, oracle, dev):
             assert np.array_equal(bad, ref_bad)
 
 
-@pytest.mark.parametrize("n", [33, 64, 75, 78, 83, 100])
+@pytest.mark.parametrize("n", [33, 64, 75, 78, 83, 90, 100])
 def test_lattice_states_with_long_lines(pkg, oracle, dev, n):
     """States built from lines, P[r] = a r + b and Q[r] = c r + d (mod n),
     without coincident points: up to n points lie on one line and most

@@ -330,6 +330,32 @@ def move_rates(pkg, torch) -> dict:
     return out
 
 
+def tempering_cpu_baseline(pkg, oracle, seconds: float = 10.0) -> dict:
+    """The oracle's replay of the reference walk (tempering.py:48-110, its
+    O(m^2) pair-loop delta) on config 3's replicas and temperatures, OpenMP
+    over replicas on all host cores: whole sweeps of 150 proposals, as many as
+    fit in about `seconds`."""
+    import numpy as np
+
+    replicas, ladder, steps = 4096, 64, 150
+    cores = oracle.cpu_count()
+    p, q = oracle.generate_states(SEED + 1, N, replicas)
+    cols = np.stack((p, q), axis=2).reshape(replicas, 2 * N)
+    temps = np.tile(np.geomspace(0.05, 2.0, ladder), replicas // ladder).astype(np.float32)
+    k_move = pkg.tempering.move_table_size(temps)
+    replay = oracle.TemperingReplay(cols, temps, 3, k_move=k_move, k_exchange=2)
+    t0 = time.perf_counter()
+    replay.advance(steps, threads=cores)
+    sweeps = 1
+    while time.perf_counter() - t0 < seconds * 0.5 and sweeps < 8:
+        replay.advance(steps, threads=cores)
+        sweeps += 1
+    elapsed = time.perf_counter() - t0
+    return {"value": sweeps * steps * replicas / elapsed, "unit": "proposals/s", "cores": cores, "kind": "port",
+            "sample": f"{sweeps} sweep(s) x {steps} proposals x {replicas} replicas (config 3's ladder), oracle C "
+                      f"replay of tempering.walk with its pair-loop delta, OpenMP over replicas"}
+
+
 def move_cpu_baseline(pkg, oracle) -> dict:
     """CPU side of the delta harness: the oracle's C loop of
     accelerated.evaluate_two_switches (one thread) on the same state and
@@ -490,6 +516,12 @@ def main() -> int:
         log("bench: CPU baseline")
         cpu = cpu_baseline()
         cpu["gpu_over_cpu"] = value / cpu["value"]
+        if "tempering_n75_64x64" in extra:
+            log("bench: tempering CPU baseline")
+            t_cpu = tempering_cpu_baseline(pkg, oracle)
+            extra["tempering_n75_64x64"]["cpu_baseline"] = t_cpu
+            extra["tempering_n75_64x64"]["gpu_over_cpu"] = (extra["tempering_n75_64x64"]["proposals_per_s"] /
+                                                           t_cpu["value"])
         if "move_scores" in extra:
             moves_cpu = move_cpu_baseline(pkg, oracle)
             extra["move_scores"]["cpu_baseline"] = moves_cpu

@@ -2,15 +2,20 @@
 // scoring with per-slot badness.
 //
 // Scoring (direction histograms).  One wavefront owns one state; lane l holds
-// slots l, l+64, l+128, ... in registers as keys row*W + column.  For every
-// anchor a, the lanes look up the direction id of (slot - a) in the LDS
-// direction table and add one to a byte-packed LDS histogram of ids; every
-// lane then reads back the count of its own id, cnt_a(b) = number of points
-// on the line (a, b) seen from a, and clears its counter.  A slot b lies in
-// sum_{a != b} (cnt_a(b) - 1) / 2 collinear triples (each triple {a, b, c}
-// is seen from a and from c), which is the reference's per-point badness
+// slots l, l+64, l+128, ... in registers as keys row*W + column, and a per-n
+// LDS table maps the key difference of two slots to the id of their
+// primitive undirected direction.
+//  - score_onesided_kernel (33 <= n <= 100): anchor a builds a histogram of
+//    the directions to the slots b < a, each b reads back its count, and
+//    every collinear triple is counted once, at its last slot (see the
+//    kernel).  C(m, 2) lookups per state.
+//  - score_slope_kernel (n <= 32, one chunk): anchor a histograms all other
+//    slots; each increment returns the slot's rank among the earlier slots
+//    of its direction, whose sum over the lanes is sum_d C(cnt_d, 2), the
+//    anchor's badness.  m(m-1) lookups per state, fewer LDS steps when the
+//    state fits one chunk.
+// Either way the result is the reference's per-point badness
 // (accelerated.py:34-36) and E = sum_b badness_b / 3 (accelerated.py:33).
-// Work per state is m(m-1) table lookups instead of C(m,3) determinants.
 //
 // States with coincident points (P[r] == Q[r]) have no direction for the
 // pair; those states take the determinant path, which counts them exactly

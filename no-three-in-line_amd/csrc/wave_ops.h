@@ -35,19 +35,6 @@ __device__ inline uint32_t wave_scan_dpp(uint32_t x) {
   return x;
 }
 
-// Minimum over the 64 lanes, by the same DPP pattern as wave_sum_dpp (lanes
-// a step does not write keep the identity 0xFFFFFFFF).
-__device__ inline uint32_t wave_min_dpp(uint32_t x) {
-  constexpr int kIdentity = -1;
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0xB1, 0xF, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x4E, 0xF, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x141, 0xF, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x140, 0xF, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x142, 0xA, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(kIdentity, (int)x, 0x143, 0xC, 0xF, false));
-  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
-}
-
 // Value of a per-lane register array at a uniform slot (slot = lane + 64 c).
 template <int CH>
 __device__ inline int slot_value(const int (&arr)[CH], int slot) {
@@ -59,22 +46,6 @@ __device__ inline int slot_value(const int (&arr)[CH], int slot) {
     r = (chunk == c) ? t : r;
   }
   return r;
-}
-
-// Values of a per-lane register array at the uniform slots 2r and 2r + 1
-// (one chunk, neighbouring lanes).
-template <int CH>
-__device__ inline void slot_pair(const int (&arr)[CH], int row, int& first, int& second) {
-  const int slot = 2 * row, lane = slot & (kWave - 1), chunk = slot >> 6;
-  int a = __builtin_amdgcn_readlane(arr[0], lane), b = __builtin_amdgcn_readlane(arr[0], lane + 1);
-#pragma unroll
-  for (int c = 1; c < CH; ++c) {
-    const int ta = __builtin_amdgcn_readlane(arr[c], lane), tb = __builtin_amdgcn_readlane(arr[c], lane + 1);
-    a = (chunk == c) ? ta : a;
-    b = (chunk == c) ? tb : b;
-  }
-  first = a;
-  second = b;
 }
 
 // Entry `index` (uniform, >= 0) of a small per-replica table `global` whose

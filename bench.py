@@ -140,7 +140,7 @@ def batched_rate(pkg, torch, sweeps: int = 1000) -> dict:
     captured; replicas per workgroup from the autotuner."""
     sizes = {40: 8192, 52: 4096, 61: 4096, 75: 2048, 83: 2048, 100: 1024}
     t0 = time.perf_counter()
-    shapes = pkg.autotune.tune(sizes, sweeps=2, candidates=(2, 4, 8, 16))
+    shapes = pkg.autotune.tune(sizes, sweeps=8, candidates=(2, 4, 8, 16), refresh=True)
     tune_seconds = time.perf_counter() - t0
     groups = []
     for i, (n, replicas) in enumerate(sizes.items()):

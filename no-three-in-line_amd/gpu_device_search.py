@@ -189,6 +189,47 @@ def _score_rank(p: torch.Tensor, q: torch.Tensor, n: int):
     return p[idx].contiguous(), q[idx].contiguous(), bad[idx].contiguous(), s_host[order], b_host[order]
 
 
+def _fill_selection(selected: list[int], candidates, elite_size: int) -> None:
+    seen = set(selected)
+    for raw in candidates:
+        index = int(raw)
+        if index not in seen:
+            selected.append(index)
+            seen.add(index)
+            if len(selected) == elite_size:
+                return
+
+
+def select_elites(scores: np.ndarray, best_score: int, energy_window: int, elite_size: int, rng: Random) -> list[int]:
+    """gpu_device_search.py:346-366 on host scores: the elite_size // 4 best
+    children (stable order), then the children within energy_window of the
+    best score in rng-shuffled order, then the best remaining ones."""
+    order = np.argsort(scores, kind="stable")
+    selected = list(map(int, order[:max(1, elite_size // 4)]))
+    diverse = list(map(int, np.flatnonzero(scores <= best_score + energy_window)))
+    rng.shuffle(diverse)
+    _fill_selection(selected, diverse, elite_size)
+    if len(selected) < elite_size:
+        _fill_selection(selected, order, elite_size)
+    return selected
+
+
+def select_elites_device(scores: torch.Tensor, best_score: int, energy_window: int, elite_size: int,
+                         rng: Random) -> list[int]:
+    """select_elites with the sort and the window search on the device: only
+    the quality share, the window's indices and (when those do not fill the
+    beam) the full order cross to the host.  Same picks and the same rng
+    calls as the host form."""
+    order = torch.sort(scores, stable=True).indices
+    selected = order[:max(1, elite_size // 4)].cpu().tolist()
+    diverse = torch.nonzero(scores <= best_score + energy_window).flatten().cpu().tolist()
+    rng.shuffle(diverse)
+    _fill_selection(selected, diverse, elite_size)
+    if len(selected) < elite_size:
+        _fill_selection(selected, order.cpu().tolist(), elite_size)
+    return selected
+
+
 def gpu_device_beam_search(initial: PairPermutations, *, seed: int, generations: int, population: int,
                            elite_size: int, max_strength: int, energy_window: int,
                            initial_elites: np.ndarray | None = None, report_every: int = 5,
@@ -230,27 +271,8 @@ def gpu_device_beam_search(initial: PairPermutations, *, seed: int, generations:
                           generation=generation, out=(child_p, child_q))
         score_population_device(child_p, child_q, child_scores, check=False)
         check_status(dev)
-        scores = child_scores.cpu().numpy()
         evaluations += population
-        order = np.argsort(scores, kind="stable")
-        selected = list(map(int, order[:max(1, elite_size // 4)]))
-        selected_set = set(selected)
-        diverse = list(map(int, np.flatnonzero(scores <= best_score + energy_window)))
-        rng.shuffle(diverse)
-        for index in diverse:
-            if index not in selected_set:
-                selected.append(index)
-                selected_set.add(index)
-                if len(selected) == elite_size:
-                    break
-        if len(selected) < elite_size:
-            for raw in order:
-                index = int(raw)
-                if index not in selected_set:
-                    selected.append(index)
-                    selected_set.add(index)
-                    if len(selected) == elite_size:
-                        break
+        selected = select_elites_device(child_scores, best_score, energy_window, elite_size, rng)
         sel = torch.from_numpy(np.asarray(selected, dtype=np.int64)).to(dev)
         elite_p, elite_q, elite_bad, elite_scores, elite_bad_host = _score_rank(
             child_p[sel].contiguous(), child_q[sel].contiguous(), n)

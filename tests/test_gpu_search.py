@@ -109,6 +109,26 @@ def test_elite_keys_and_device_ranking(pkg, oracle, dev):
     assert np.array_equal(order, pkg.rank_elites(scores, bad))
 
 
+def test_device_elite_selection_matches_host(pkg, dev):
+    """The device sort and window search pick the same children, in the same
+    order and with the same rng calls, as the host argsort form, with many
+    ties and windows larger and smaller than the beam."""
+    from random import Random
+
+    rng = np.random.default_rng(4)
+    for trial in range(12):
+        population = int(rng.integers(50, 5000))
+        scores = rng.integers(0, int(rng.integers(3, 60)), size=population).astype(np.int32)
+        best = int(scores.min()) + int(rng.integers(0, 3))
+        window = int(rng.integers(0, 4))
+        elite = int(rng.integers(1, min(population, 300)))
+        host = pkg.gpu_device_search.select_elites(scores, best, window, elite, Random(trial))
+        device = pkg.gpu_device_search.select_elites_device(torch.from_numpy(scores).to(dev), best, window, elite,
+                                                            Random(trial))
+        assert device == host
+        assert len(host) >= min(elite, population) and len(set(host)) == len(host)
+
+
 def test_device_beam_search_runs_and_improves(pkg, oracle, dev):
     from random import Random
 

@@ -83,6 +83,12 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// v_writelane_b32: `old` with lane `lane` (uniform, 0..63) replaced by the
+// uniform `value`.  Clang has no builtin for it; this binds the LLVM
+// intrinsic, so the compiler still tracks the instruction's hazards.
+extern "C" __device__ int llvm_amdgcn_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+__device__ inline int write_lane(int value, int lane, int old) { return llvm_amdgcn_writelane(value, lane, old); }
+
 __device__ inline int wave_sum(int x) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);

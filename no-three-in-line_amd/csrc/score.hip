@@ -394,13 +394,16 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
     hist_base = (uint32_t)(tab.bytes + wave * tab.hist_words * 4);
   }
   uint32_t* hist = reinterpret_cast<uint32_t*>(lds + hist_base);
-  // counter word of an entry and the counter's bit position in it
-  auto counter = [&](uint32_t e) -> uint32_t* {
-    if constexpr (kLayout == kWide32) return reinterpret_cast<uint32_t*>(lds + hist_base + (e >> 16));
-    if constexpr (kLayout == kNibble16) return reinterpret_cast<uint32_t*>(lds + hist_base + ((e >> 3) & ~3u));
-    if constexpr (kLayout == kPacked16) return reinterpret_cast<uint32_t*>(lds + (hist_base | (e & ~3u)));
-    return reinterpret_cast<uint32_t*>(lds + hist_base + (e & ~3u));
+  // byte offset (from lds) of an entry's counter word, and the counter's
+  // bit position in it
+  auto counter = [&](uint32_t e) -> uint32_t {
+    if constexpr (kLayout == kWide32) return hist_base + (e >> 16);
+    if constexpr (kLayout == kNibble16) return hist_base + ((e >> 3) & ~3u);
+    if constexpr (kLayout == kPacked16) return hist_base | (e & ~3u);
+    return hist_base + (e & ~3u);
   };
+  using LdsWord = __attribute__((address_space(3))) uint32_t;
+  auto word_at = [&](uint32_t offset) { return (LdsWord*)reinterpret_cast<uint32_t*>(lds + offset); };
   // the low five bits are the shift; v_lshlrev and v_bfe read only those
   auto shift = [](uint32_t e) -> uint32_t {
     if constexpr (kLayout == kWide32 || kLayout == kNibble16) return e;
@@ -492,36 +495,45 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
     // in chunk ca the lanes before l are.  The next anchor's lookups (table
     // only) issue together with this anchor's read-backs and clears, so each
     // anchor waits on LDS once.
+    // lane < l as a scalar lane mask (l is uniform), so the tests below cost
+    // no vector compare
+    auto before = [](int l) { return __builtin_amdgcn_inverse_ballot_w64((1ull << l) - 1ull); };
     auto lookup = [&](int ca, int l, uint32_t(&e)[CH]) {
       const unsigned char* from = lds + (zero - __builtin_amdgcn_readlane(at[ca], l));
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
-        const unsigned char* entry = (c < ca || lane < l) ? from + at[c] : idle_entry;
+        const unsigned char* entry = (c < ca || before(l)) ? from + at[c] : idle_entry;
         e[c] = *reinterpret_cast<const Entry*>(entry);
       }
     };
     // 4-bit counters: a lane whose increment finds 15 would carry into the
     // next counter; it flags the state for the determinant recount
     bool overflow = false;
-    auto increment = [&](int ca, int l, const uint32_t(&e)[CH]) {
+    // ctr[c]: the lane's counter word, computed once for the increment and
+    // the read-back (the empty asm keeps the compiler from recomputing it on
+    // both sides of the tail chunk's mask)
+    auto increment = [&](int ca, int l, const uint32_t(&e)[CH], LdsWord* (&ctr)[CH]) {
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
-        if (c < ca || lane < l) {
+        ctr[c] = word_at(counter(e[c]));
+        asm volatile("" : "+v"(ctr[c]));
+        const uint32_t one = 1u << (shift(e[c]) & 31u);
+        if (c < ca || before(l)) {
           if constexpr (kLayout == kNibble16) {
-            const uint32_t old = atomicAdd(counter(e[c]), 1u << (shift(e[c]) & 31u));
+            const uint32_t old = __hip_atomic_fetch_add(ctr[c], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             overflow |= __builtin_amdgcn_ubfe(old, shift(e[c]), 4) == 15u;
           } else {
-            atomicAdd(counter(e[c]), 1u << (shift(e[c]) & 31u));
+            __hip_atomic_fetch_add(ctr[c], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         }
       }
       wave_sync();
     };
-    auto collect = [&](int ca, const uint32_t(&e)[CH], uint32_t(&word)[CH]) {
+    auto collect = [&](int ca, LdsWord* const (&ctr)[CH], uint32_t(&word)[CH]) {
 #pragma unroll
-      for (int c = 0; c <= ca; ++c) word[c] = *counter(e[c]);
+      for (int c = 0; c <= ca; ++c) word[c] = *ctr[c];
 #pragma unroll
-      for (int c = 0; c <= ca; ++c) *counter(e[c]) = 0u;
+      for (int c = 0; c <= ca; ++c) *ctr[c] = 0u;
     };
     // sum of the lanes' counts; each slot keeps its own running sum
     auto accumulate = [&](int ca, const uint32_t(&e)[CH], const uint32_t(&word)[CH]) {
@@ -540,18 +552,19 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
       const int begin = (ca == 0) ? 1 : 0;
       const int limit = min(kWave, m - kWave * ca);
       uint32_t e0[CH], e1[CH], word0[CH], word1[CH];
+      LdsWord* ctr[CH];
       if (begin < limit) lookup(ca, begin, e0);
       wave_sync();
       for (int l = begin; l < limit; l += 2) {
-        increment(ca, l, e0);
-        collect(ca, e0, word0);
+        increment(ca, l, e0, ctr);
+        collect(ca, ctr, word0);
         if (l + 1 < limit) lookup(ca, l + 1, e1);
         wave_sync();
         const uint32_t p0 = accumulate(ca, e0, word0);
         uint32_t p1 = 0;
         if (l + 1 < limit) {
-          increment(ca, l + 1, e1);
-          collect(ca, e1, word1);
+          increment(ca, l + 1, e1, ctr);
+          collect(ca, ctr, word1);
           if (l + 2 < limit) lookup(ca, l + 2, e0);
           wave_sync();
           p1 = accumulate(ca, e1, word1);
@@ -563,8 +576,10 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
         const int a = kWave * ca + l;
         const int first_own = (int)((total & 0xFFFFu) - (uint32_t)a) >> 1;
         const int second_own = (int)((total >> 16) - (uint32_t)(a + 1)) >> 1;
-        own[ca] = (lane == l) ? first_own : own[ca];
-        own[ca] = (lane == l + 1) ? second_own : own[ca];
+        // second first: when l + 1 is past the chunk (slot m or lane 64,
+        // clamped to 63) the first write then restores lane l's value
+        own[ca] = write_lane(second_own, min(l + 1, kWave - 1), own[ca]);
+        own[ca] = write_lane(first_own, l, own[ca]);
       }
     }
     if constexpr (kLayout == kNibble16) {

@@ -152,17 +152,25 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     const int dd = (cu == ref_u0 || cu == ref_u1) + (cv == ref_v0 || cv == ref_v1) - (cv == ref_u0 || cv == ref_u1) -
                    (cu == ref_v0 || cu == ref_v1);
 
-    // joint(u, v) in the current state
-    const int id_uv = table[key_v - key_u + offset];
-    int idu[CH];
     bool others[CH];
-    int joint_old = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
       others[c] = (k < m) && k != u && k != v;
-      idu[c] = table[key[c] - key_u + offset];
     }
+    // joint(u, v) in the current state: the points other than u, v on the
+    // line (u, v).  Only the exact delta needs it; when the bound below
+    // already rejects, the lookups are skipped.
+    int id_uv = 0, idu[CH], joint_old = 0;
+    auto old_joint = [&]() {
+      id_uv = table[key_v - key_u + offset];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) idu[c] = table[key[c] - key_u + offset];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) joint_old += __popcll(__ballot(others[c] && idu[c] == id_uv));
+    };
+    // the log records the exact delta of every valid proposal
+    if (log_base) old_joint();
     // histograms seen from u' and v' in the moved state.  The rejection path
     // needs only bad'(u') and bad'(v') = sum_d C(cnt_d, 2), the sums of the
     // ranks the increments return, so both histograms are built and cleared
@@ -179,11 +187,6 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       const int moved = (k == v) ? key_v2 : ((k == u) ? key_u2 : key[c]);
       idu2[c] = table[moved - key_u2 + offset];
       idv2[c] = table[moved - key_v2 + offset];
-    }
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      // the masks are not kept: the accepted path compares again per lane
-      joint_old += __popcll(__ballot(others[c] && idu[c] == id_uv));
     }
     uint32_t word_u2[CH], word_v2[CH], old_u2[CH], old_v2[CH];
 #pragma unroll
@@ -221,6 +224,14 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     const uint32_t packed = wave_sum_dpp(part);
     const int bad_u2 = (int)(packed & 0xFFFFu), bad_v2 = (int)(packed >> 16);
     const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
+    if (!log_base) {
+      // delta >= delta - joint(u, v), and the Metropolis words floor(2^32
+      // exp(-d / T)) do not grow with d: a proposal that the bound rejects
+      // is rejected by the exact delta too
+      const int low = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v);
+      if (low > 0 && r[2] >= register_table(thr_lo, thr_hi, thr, low < a.k_move ? low : a.k_move - 1)) continue;
+      old_joint();
+    }
     const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
 
     // Metropolis on the integer delta (tempering.py:91-95)

@@ -235,8 +235,11 @@ def search_rates(pkg) -> dict:
     from random import Random
 
     state = pkg.random_state(75, Random(7))
-    pkg.gpu_device_beam_search(state, seed=1, generations=2, population=4096, elite_size=64, max_strength=8,
-                               energy_window=2)  # warm-up: code objects, allocator
+    # warm-up at the timed sizes: code objects (the library sorts pick their
+    # kernels by size), allocator
+    pkg.gpu_device_beam_search(state, seed=1, generations=2, population=65536, elite_size=256, max_strength=8,
+                               energy_window=2)
+    pkg.gpu_exhaustive_beam_search(state, seed=1, generations=1, elite_size=32, energy_window=1)
     ga = pkg.gpu_device_beam_search(state, seed=1, generations=20, population=65536, elite_size=256,
                                     max_strength=8, energy_window=2)
     beam = pkg.gpu_exhaustive_beam_search(state, seed=1, generations=5, elite_size=32, energy_window=1)

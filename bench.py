@@ -354,9 +354,21 @@ def tempering_cpu_baseline(pkg, oracle, seconds: float = 10.0) -> dict:
         replay.advance(steps, threads=cores)
         sweeps += 1
     elapsed = time.perf_counter() - t0
+    # the device walk over the same sweeps, states, temperatures and seed:
+    # every replica's trajectory must match the replay's
+    import torch
+
+    chains = pkg.TemperingChains(cols, temps, seed=3)
+    for _ in range(sweeps):
+        chains.advance(steps)
+    torch.cuda.synchronize()
+    exact = all(np.array_equal(getattr(chains, name).cpu().numpy(), getattr(replay, name))
+                for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted",
+                             "removed_points"))
     return {"value": sweeps * steps * replicas / elapsed, "unit": "proposals/s", "cores": cores, "kind": "port",
             "sample": f"{sweeps} sweep(s) x {steps} proposals x {replicas} replicas (config 3's ladder), oracle C "
-                      f"replay of tempering.walk with its pair-loop delta, OpenMP over replicas"}
+                      f"replay of tempering.walk with its pair-loop delta, OpenMP over replicas",
+            "trajectory_exact": exact}
 
 
 def move_cpu_baseline(pkg, oracle) -> dict:
@@ -509,7 +521,7 @@ def main() -> int:
         import numpy as np
 
         oracle = entry.load_oracle()
-        idx = np.arange(0, POPULATION, POPULATION // 64)
+        idx = np.arange(0, POPULATION, POPULATION // 512)  # the 512-state subsample
         hp, hq = p[idx].cpu().numpy(), q[idx].cpu().numpy()
         ref_s, ref_b = oracle.score_population(hp, hq)
         exact = bool(np.array_equal(scores[idx].cpu().numpy(), ref_s) and
@@ -525,6 +537,9 @@ def main() -> int:
             extra["tempering_n75_64x64"]["cpu_baseline"] = t_cpu
             extra["tempering_n75_64x64"]["gpu_over_cpu"] = (extra["tempering_n75_64x64"]["proposals_per_s"] /
                                                            t_cpu["value"])
+            if not t_cpu["trajectory_exact"]:
+                log("bench: PARITY FAILURE in the tempering trajectories")
+                exact = False
         if "move_scores" in extra:
             moves_cpu = move_cpu_baseline(pkg, oracle)
             extra["move_scores"]["cpu_baseline"] = moves_cpu

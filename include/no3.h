@@ -118,7 +118,8 @@ typedef struct n3_chains {
     int32_t* accepted;       /* [R] */
     const int32_t* reference;/* [2n] umbrella reference state */
     int32_t* removed_points; /* [R] */
-    const uint32_t* move_thresholds;  /* [R][k_move]: floor(2^32 exp(-d/T_r)) */
+    const uint32_t* move_thresholds;  /* [R][k_move]: floor(2^32 exp(-d/T_r)), non-increasing in d (the walk's
+                                         early rejection relies on it) */
     int k_move;
     const uint32_t* exchange_thresholds; /* [R][k_exchange] for pair (r, r+1) */
     const int8_t* exchange_sign;         /* [R] sign of 1/T_r - 1/T_{r+1} */

@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
         const int k = lane + kWave * c;
         part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
                 __builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8);
-        joint_new += __popcll(__ballot((k < m) && k != u && k != v && idu2[c] == id_u2v2));
+        joint_new += __popcll(lanes((k < m) && k != u && k != v) & lanes(idu2[c] == id_u2v2));
       }
       // bad'(u') + bad'(v') (each below 2^16, so the sum fits one word)
       const long long new_affected = (long long)wave_sum_dpp(part) - joint_new;
@@ -235,9 +235,9 @@ __global__ void __launch_bounds__(256) three_cycle_kernel(const int32_t* __restr
       const bool live = k < m;
       const int a = table[key[c] - key_u + offset];
       const int b = table[key[c] - key_v + offset];
-      joint += __popcll(__ballot(live && k != u && k != v && (a == id_uv || a == zero_id)));
-      joint += __popcll(__ballot(live && k != u && k != w && (a == id_uw || a == zero_id)));
-      joint += __popcll(__ballot(live && k != v && k != w && (b == id_vw || b == zero_id)));
+      joint += __popcll(lanes(live && k != u && k != v) & lanes(a == id_uv || a == zero_id));
+      joint += __popcll(lanes(live && k != u && k != w) & lanes(a == id_uw || a == zero_id));
+      joint += __popcll(lanes(live && k != v && k != w) & lanes(b == id_vw || b == zero_id));
     }
     const int triple = (key_v != key_u && key_w != key_u && id_uv == id_uw) ? 1 : 0;
     const long long old_affected = badness[u] + badness[v] + badness[w] - joint + triple;
@@ -297,9 +297,9 @@ __global__ void __launch_bounds__(256) three_cycle_kernel(const int32_t* __restr
       const bool live = k < m;
       part_uv += rank_of(rank_u[c], idu2[c], k, u) + (rank_of(rank_v[c], idv2[c], k, v) << 16);
       part_w += rank_of(rank_w[c], idw2[c], k, w);
-      joint_new += __popcll(__ballot(live && k != u && k != v && idu2[c] == id2_uv));
-      joint_new += __popcll(__ballot(live && k != u && k != w && idu2[c] == id2_uw));
-      joint_new += __popcll(__ballot(live && k != v && k != w && idv2[c] == id2_vw));
+      joint_new += __popcll(lanes(live && k != u && k != v) & lanes(idu2[c] == id2_uv));
+      joint_new += __popcll(lanes(live && k != u && k != w) & lanes(idu2[c] == id2_uw));
+      joint_new += __popcll(lanes(live && k != v && k != w) & lanes(idv2[c] == id2_vw));
     }
     const uint32_t packed = wave_sum_dpp(part_uv), total_w = wave_sum_dpp(part_w);
     const int triple_new = (id2_uv == id2_uw) ? 1 : 0;

@@ -173,8 +173,8 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const bool live = lane + kWave * c < m;
-        below_u += __popcll(__ballot(live && cdf[c] <= target_u));
-        below_v += __popcll(__ballot(live && cdf[c] <= target_v));
+        below_u += __popcll(lanes(live) & lanes(cdf[c] <= target_u));
+        below_v += __popcll(lanes(live) & lanes(cdf[c] <= target_v));
       }
       u = below_u < m - 1 ? below_u : m - 1;
       v = below_v < m - 1 ? below_v : m - 1;
@@ -213,7 +213,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     }
     int joint_old = 0;
 #pragma unroll
-    for (int c = 0; c < CH; ++c) joint_old += __popcll(__ballot(others[c] && idu[c] == id_uv));
+    for (int c = 0; c < CH; ++c) joint_old += __popcll(lanes(others[c]) & lanes(idu[c] == id_uv));
     uint32_t word_u2[CH], word_v2[CH], old_u2[CH], old_v2[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
@@ -241,7 +241,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     for (int c = 0; c < CH; ++c) {
       part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
               (__builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16);
-      joint_new += __popcll(__ballot(others[c] && idu2[c] == id_u2v2));
+      joint_new += __popcll(lanes(others[c]) & lanes(idu2[c] == id_u2v2));
     }
     const uint32_t packed = wave_sum_dpp(part);
     const int bad_u2 = (int)(packed & 0xFFFFu), bad_v2 = (int)(packed >> 16);

@@ -89,6 +89,12 @@ __device__ inline void wave_sync() {
 extern "C" __device__ int llvm_amdgcn_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 __device__ inline int write_lane(int value, int lane, int old) { return llvm_amdgcn_writelane(value, lane, old); }
 
+// Lane mask of `pred`.  __ballot(int) of a compound predicate makes the
+// compiler turn the predicate into a 0/1 vector and compare it again (two
+// more VALU instructions per call); masks of the parts, AND-ed on the scalar
+// unit, avoid that.
+__device__ inline uint64_t lanes(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }
+
 __device__ inline int wave_sum(int x) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) mutate_kernel(const int32_t* __restrict__
           const uint32_t target = mulhi32(b[1], total);
           int below = 0;
 #pragma unroll
-          for (int c = 0; c < CH; ++c) below += __popcll(__ballot(lane + kWave * c < m && cdf[c] <= target));
+          for (int c = 0; c < CH; ++c) below += __popcll(lanes(lane + kWave * c < m) & lanes(cdf[c] <= target));
           u = below < m - 1 ? below : m - 1;
         }
         const int v = (int)mulhi32(b[3], (uint32_t)m);

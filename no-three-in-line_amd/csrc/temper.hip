@@ -132,7 +132,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
         // inclusive prefix of badness over the slots of this chunk
         const int x = (int)wave_scan_dpp((uint32_t)bad[c]);
         const int k = lane + kWave * c;
-        below += __popcll(__ballot(k < m && carry + x <= target));
+        below += __popcll(lanes(k < m) & lanes(carry + x <= target));
         carry += __builtin_amdgcn_readlane(x, 63);
       }
       u = below < m - 1 ? below : m - 1;
@@ -167,7 +167,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
 #pragma unroll
       for (int c = 0; c < CH; ++c) idu[c] = table[key[c] - key_u + offset];
 #pragma unroll
-      for (int c = 0; c < CH; ++c) joint_old += __popcll(__ballot(others[c] && idu[c] == id_uv));
+      for (int c = 0; c < CH; ++c) joint_old += __popcll(lanes(others[c]) & lanes(idu[c] == id_uv));
     };
     // the log records the exact delta of every valid proposal
     if (log_base) old_joint();
@@ -218,7 +218,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       // the moved slot return rank 0
       part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
               (__builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16);
-      joint_new += __popcll(__ballot(others[c] && idu2[c] == id_u2v2));
+      joint_new += __popcll(lanes(others[c]) & lanes(idu2[c] == id_u2v2));
     }
     // bad'(u') in the low half, bad'(v') in the high half (each < 2^16)
     const uint32_t packed = wave_sum_dpp(part);

@@ -7,10 +7,12 @@ state's energy.  A move's score is the sum of its two points' scores and the
 network is trained online, with the hand-written backward kernel, on the
 (move, accepted, delta) tuples that plain uniform-proposal runs log on the
 device.  For sampling, the network is tabulated over (temperature, energy,
-count) into integer weights; the walk draws the first point from
-0.2 uniform + 0.8 weights, the second uniformly, and corrects with the
-Hastings ratio, so the exact integer delta stays the judge of acceptance
-(csrc/learned.hip).
+count) into integer weights w.  With probability 0.2 the walk proposes a
+uniform pair of slots; otherwise it draws both slots in proportion to w (by
+the inverse CDF), which samples the softmax over all pairs exactly, since
+the pair score is additive.  The Hastings ratio of the forward and reverse
+proposal probabilities corrects the acceptance test, and the exact integer
+delta still decides acceptance (csrc/learned.hip).
 """
 from __future__ import annotations
 

@@ -134,6 +134,15 @@ typedef struct n3_chains {
                                 and reservations past the capacity write nothing */
     int64_t log_capacity;    /* entries */
     uint64_t* log_count;     /* entries reserved (may exceed the capacity) */
+    /* umbrella potentials (tempering.py:92-95; NULL strengths = none): a
+       move's energy is E = delta + strengths[r] (2 (removed_r - centers[r]) dd
+       + dd^2), dd the change of removed_points, and it is accepted when
+       E <= 0 or its 32-bit word is below floor(2^32 exp(-E inverse_temperatures[r])).
+       Accepted by n3_temper_advance and by n3_temper_run without exchanges
+       only (the reference disables exchanges with umbrellas, tempering.py:173-174). */
+    const float* centers;                /* [R] */
+    const float* strengths;              /* [R] or NULL */
+    const double* inverse_temperatures;  /* [R]: 1 / (double)T_r, required with strengths */
 } n3_chains;
 
 /* `steps` proposals per replica (one sweep), TemperingChains.advance

@@ -55,6 +55,9 @@ class Chains(ctypes.Structure):
         ("log", ctypes.c_void_p),
         ("log_capacity", ctypes.c_int64),
         ("log_count", ctypes.c_void_p),
+        ("centers", ctypes.c_void_p),
+        ("strengths", ctypes.c_void_p),
+        ("inverse_temperatures", ctypes.c_void_p),
     ]
 
 

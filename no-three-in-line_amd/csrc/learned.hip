@@ -441,6 +441,8 @@ extern "C" int n3_temper_run_learned(const n3_chains* chains, const uint32_t* we
   if (st) return st;
   if (!weights || !temp_index || !thr64 || max_energy < 0 || k64 < 2 || steps < 1 || sweeps < 0)
     return set_error(N3_EINVAL, "bad learned-walk arguments");
+  if (chains->strengths)
+    return set_error(N3_EINVAL, "umbrella potentials are supported by n3_temper_advance and n3_temper_run only");
   if (ladder > 1) {
     st = check_exchange(chains, ladder);
     if (st) return st;

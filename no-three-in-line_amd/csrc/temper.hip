@@ -35,9 +35,48 @@
 namespace no3 {
 
 
+// floor(2^32 exp(-x)) for x > 0, from operations that every IEEE-754 double
+// unit rounds alike (fma, rint, ldexp, one product), so that a host replay
+// computes the same word: Cody-Waite reduction y = -x = k ln2 + f with
+// fdlibm's two-part ln2 (|f| <= ln2 / 2), then the Taylor series of exp(f)
+// to degree 13 (truncation error below 1e-17) by Horner's rule.
+__device__ inline uint32_t exp_word(double x) {
+#pragma clang fp contract(off)
+  if (!(x < 23.0)) return 0u;  // 2^32 exp(-23) < 1
+  const double y = -x;
+  const double k = rint(y * 1.4426950408889634);
+  double f = fma(k, -6.93147180369123816490e-01, y);
+  f = fma(k, -1.90821492927058770002e-10, f);
+  double p = 1.0 / 6227020800.0;
+  p = fma(p, f, 1.0 / 479001600.0);
+  p = fma(p, f, 1.0 / 39916800.0);
+  p = fma(p, f, 1.0 / 3628800.0);
+  p = fma(p, f, 1.0 / 362880.0);
+  p = fma(p, f, 1.0 / 40320.0);
+  p = fma(p, f, 1.0 / 5040.0);
+  p = fma(p, f, 1.0 / 720.0);
+  p = fma(p, f, 1.0 / 120.0);
+  p = fma(p, f, 1.0 / 24.0);
+  p = fma(p, f, 1.0 / 6.0);
+  p = fma(p, f, 0.5);
+  p = fma(p, f, 1.0);
+  p = fma(p, f, 1.0);
+  const double v = ldexp(p, (int)k + 32);
+  return v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
+}
+
+// Umbrella energy of a move (tempering.py:93-94): delta + strength (2 offset
+// dd + dd^2) with offset = removed - center, rounded the same way on the host.
+__device__ inline double umbrella_energy(int delta, int dd, int removed, float strength, float center) {
+#pragma clang fp contract(off)
+  const double offset = (double)removed - (double)center;
+  const double a = fma(2.0 * offset, (double)dd, (double)(dd * dd));
+  return fma((double)strength, a, (double)delta);
+}
+
 // One replica slot's `a.steps` proposals; `table` and `hist` are this
 // wavefront's LDS direction table and histogram.
-template <int CH>
+template <int CH, bool kUmbrella = false>
 __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t* hist, const DirTableInfo& info,
                              int slot_in) {
   const int lane = lane_id();
@@ -82,6 +121,15 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
   const bool tail_in = lane + kWave * (CH - 1) < m;
   int score = a.scores[slot];
   int removed = a.removed[slot];
+  // umbrella potential of this slot (tempering.py:92-95), uniform values;
+  // a separate instantiation, so the plain walk keeps its registers
+  float strength = 0.0f, center = 0.0f;
+  double inverse_t = 0.0;
+  if constexpr (kUmbrella) {
+    strength = a.strengths[slot];
+    center = a.centers[slot];
+    inverse_t = a.inverse_temperatures[slot];
+  }
   int best = a.best_scores[slot];
   int accepted = 0;
   unsigned long long valid_moves = 0;
@@ -169,8 +217,10 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
 #pragma unroll
       for (int c = 0; c < CH; ++c) joint_old += __popcll(lanes(others[c]) & lanes(idu[c] == id_uv));
     };
-    // the log records the exact delta of every valid proposal
-    if (log_base) old_joint();
+    // the log records the exact delta of every valid proposal, and the
+    // umbrella test has no integer bound
+    const bool exact = log_base != nullptr || kUmbrella;
+    if (exact) old_joint();
     // histograms seen from u' and v' in the moved state.  The rejection path
     // needs only bad'(u') and bad'(v') = sum_d C(cnt_d, 2), the sums of the
     // ranks the increments return, so both histograms are built and cleared
@@ -224,7 +274,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     const uint32_t packed = wave_sum_dpp(part);
     const int bad_u2 = (int)(packed & 0xFFFFu), bad_v2 = (int)(packed >> 16);
     const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
-    if (!log_base) {
+    if (!exact) {
       // delta >= delta - joint(u, v), and the Metropolis words floor(2^32
       // exp(-d / T)) do not grow with d: a proposal that the bound rejects
       // is rejected by the exact delta too
@@ -235,8 +285,14 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     const int delta = (bad_u2 + bad_v2 - joint_new) - (bad_u + bad_v - joint_old);
 
     // Metropolis on the integer delta (tempering.py:91-95)
-    bool accept = delta <= 0;
-    if (!accept) accept = r[2] < register_table(thr_lo, thr_hi, thr, delta < a.k_move ? delta : a.k_move - 1);
+    bool accept;
+    if constexpr (kUmbrella) {
+      const double e = umbrella_energy(delta, dd, removed, strength, center);
+      accept = e <= 0.0 || r[2] < exp_word(e * inverse_t);
+    } else {
+      accept = delta <= 0;
+      if (!accept) accept = r[2] < register_table(thr_lo, thr_hi, thr, delta < a.k_move ? delta : a.k_move - 1);
+    }
     if (log_base && lane == 0) {
       // (badness u, badness v, energy, slot, delta, accepted): training data
       // for the proposal network
@@ -314,7 +370,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
   }
 }
 
-template <int CH>
+template <int CH, bool kUmbrella = false>
 __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* __restrict__ table_g,
                                                    DirTableInfo info) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -326,7 +382,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
   const int slot = blockIdx.x * waves + wave;
-  if (slot < a.replicas) walk_replica<CH>(a, table, hist, info, slot);
+  if (slot < a.replicas) walk_replica<CH, kUmbrella>(a, table, hist, info, slot);
 }
 
 // Several populations of different sizes in one launch: block b belongs to
@@ -519,6 +575,8 @@ int check_chains(const n3_chains* c) {
       !c->reference || !c->removed_points || !c->move_thresholds || !c->counters || !c->proposals)
     return set_error(N3_EINVAL, "null chains array");
   if (c->k_move < 2) return set_error(N3_EINVAL, "k_move must be at least 2");
+  if (c->strengths && (!c->centers || !c->inverse_temperatures))
+    return set_error(N3_EINVAL, "umbrella strengths need centers and inverse temperatures");
   return N3_OK;
 }
 
@@ -531,7 +589,7 @@ struct WalkPlan {
   void (*kernel)(WalkArgs, const uint16_t*, DirTableInfo);
 };
 
-static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan) {
+static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan, bool umbrella = false) {
   int st = device_dirtable(n, &plan->table, &plan->info);
   if (st) return st;
   int lds_limit = 0, compute_units = 0;
@@ -555,10 +613,10 @@ static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan) {
   plan->lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
   const int ch = (2 * n + kWave - 1) / kWave;
   switch (ch) {
-    case 1: plan->kernel = walk_kernel<1>; break;
-    case 2: plan->kernel = walk_kernel<2>; break;
-    case 3: plan->kernel = walk_kernel<3>; break;
-    default: plan->kernel = walk_kernel<4>; break;
+    case 1: plan->kernel = umbrella ? walk_kernel<1, true> : walk_kernel<1>; break;
+    case 2: plan->kernel = umbrella ? walk_kernel<2, true> : walk_kernel<2>; break;
+    case 3: plan->kernel = umbrella ? walk_kernel<3, true> : walk_kernel<3>; break;
+    default: plan->kernel = umbrella ? walk_kernel<4, true> : walk_kernel<4>; break;
   }
   return N3_OK;
 }
@@ -567,7 +625,8 @@ WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed) {
   return WalkArgs{c->n, c->replicas, steps, bad_bias, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
                   c->scores, c->best_columns, c->best_scores, c->accepted, c->reference, c->removed_points,
                   c->move_thresholds, c->k_move, c->counters, reinterpret_cast<unsigned long long*>(c->proposals),
-                  c->log, (long long)c->log_capacity, reinterpret_cast<unsigned long long*>(c->log_count)};
+                  c->log, (long long)c->log_capacity, reinterpret_cast<unsigned long long*>(c->log_count),
+                  c->centers, c->strengths, c->inverse_temperatures};
 }
 
 // The walk; with bump_sweep the sweep counter is advanced by a counter kernel
@@ -604,6 +663,8 @@ int check_exchange(const n3_chains* c, int ladder) {
   if (ladder < 2 || c->replicas % ladder) return set_error(N3_EINVAL, "ladder_size must divide replica count");
   if (!c->exchange_thresholds || !c->exchange_sign || c->k_exchange < 2)
     return set_error(N3_EINVAL, "exchange thresholds missing");
+  // tempering.py:173-174
+  if (c->strengths) return set_error(N3_EINVAL, "temperature-only exchange is disabled with umbrella potentials");
   return N3_OK;
 }
 
@@ -781,6 +842,8 @@ extern "C" int n3_temper_batched_run(const n3_group* groups, int group_count, in
   for (int i = 0; i < group_count; ++i) {
     int st = check_chains(&groups[i].chains);
     if (st) return st;
+    if (groups[i].chains.strengths)
+      return set_error(N3_EINVAL, "umbrella potentials are supported by n3_temper_advance and n3_temper_run only");
     if (groups[i].steps < 1 || groups[i].bad_bias < 0 || groups[i].bad_bias > 100)
       return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
     if (groups[i].ladder > 1) {
@@ -850,7 +913,7 @@ extern "C" int n3_temper_advance(const n3_chains* chains, int steps, int bad_bia
   if (steps < 1 || bad_bias < 0 || bad_bias > 100)
     return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
   WalkPlan plan;
-  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan);
+  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan, chains->strengths != nullptr);
   if (st) return st;
   return enqueue_advance(chains, plan, steps, bad_bias, seed, as_stream(stream));
 }
@@ -876,7 +939,7 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
   }
   if (sweeps == 0) return N3_OK;
   WalkPlan plan;
-  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan);
+  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan, chains->strengths != nullptr);
   if (st) return st;
   hipStream_t s = as_stream(stream);
   const int per_graph = sweeps < 32 ? sweeps : 32;

@@ -26,6 +26,9 @@ struct WalkArgs {
   int32_t* log;                   // optional proposal log, 6 int32 per entry
   long long log_capacity;
   unsigned long long* log_count;
+  const float* centers;           // umbrella potentials: null strengths = none
+  const float* strengths;
+  const double* inverse_temperatures;
 };
 
 }  // namespace no3

@@ -13,8 +13,11 @@ Differences from the reference, all deliberate:
     instead of log(U) < -x in double;
   - exchange runs on the device (no host round trip) and, like the
     reference's, swaps the replicas' states between fixed-temperature slots;
-  - umbrella potentials (strengths != 0, --umbrella-max) are not supported
-    on the device and raise NotImplementedError.
+  - umbrella potentials (strengths != 0, --umbrella-max) run on the device
+    walk with the reference's energy delta + s (2 (removed - center) dd +
+    dd^2), accepted against floor(2^32 exp(-E / T)) computed by a fixed
+    sequence of fma operations (so the CPU replay reproduces it); the batched
+    and learned walks do not take them (NotImplementedError).
 """
 from __future__ import annotations
 
@@ -98,6 +101,8 @@ class TemperingChains:
             dtype=torch.int32, device=dev)
         self.centers = torch.zeros(len(columns), dtype=torch.float32, device=dev)
         self.strengths = torch.zeros(len(columns), dtype=torch.float32, device=dev)
+        # 1 / T in double for the umbrella acceptance (host division, as in the replay)
+        self.inverse_temperatures = torch.from_numpy(1.0 / temperatures.astype(np.float64)).to(dev)
         self.k_move = move_table_size(temperatures)
         self.move_thresholds = torch.from_numpy(metropolis_thresholds(temperatures, self.k_move)).to(dev)
         self._exchange_tables: dict[int, tuple[torch.Tensor, torch.Tensor, int]] = {}
@@ -154,11 +159,28 @@ class TemperingChains:
             c.log = self.log.entries.data_ptr()
             c.log_capacity = self.log.capacity
             c.log_count = self.log.count.data_ptr()
+        if self._umbrella():
+            c.centers = self.centers.data_ptr()
+            c.strengths = self.strengths.data_ptr()
+            c.inverse_temperatures = self.inverse_temperatures.data_ptr()
         return c
 
+    def _umbrella(self) -> bool:
+        """Whether any slot has an umbrella potential; the arrays are made
+        float32, contiguous and resident on this device (in place, so that
+        captured graphs keep valid pointers)."""
+        for name in ("centers", "strengths"):
+            t = getattr(self, name)
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.device != self.device or \
+                    t.shape != (len(self.temperatures_host),):
+                setattr(self, name, torch.as_tensor(t, dtype=torch.float32, device=self.device)
+                        .reshape(len(self.temperatures_host)).contiguous())
+        return bool(torch.any(self.strengths != 0).item())
+
     def _check_umbrella(self) -> None:
-        if bool(torch.any(self.strengths != 0).item()):
-            raise NotImplementedError("umbrella potentials are not supported by the device walk")
+        """The batched and learned walks have no umbrella term."""
+        if self._umbrella():
+            raise NotImplementedError("umbrella potentials are supported by advance() and run() only")
 
     @property
     def sweep(self) -> int:
@@ -176,7 +198,6 @@ class TemperingChains:
         """tempering.py:158-167: `steps` proposals per replica (synchronous)."""
         if steps < 1 or not 0 <= bad_bias <= 100:
             raise ValueError("steps must be positive and bad_bias in [0,100]")
-        self._check_umbrella()
         chains = self._chains()
         _lib.check(_lib.lib().n3_temper_advance(ctypes.byref(chains), int(steps), int(bad_bias), self.seed,
                                                 self._stream()))
@@ -186,7 +207,7 @@ class TemperingChains:
         """tempering.py:169-191: neighbouring-temperature swaps inside each ladder."""
         if ladder_size < 2 or len(self.temperatures_host) % ladder_size:
             raise ValueError("ladder_size must divide replica count")
-        if bool(torch.any(self.strengths != 0).item()):
+        if self._umbrella():
             raise ValueError("temperature-only exchange is disabled with umbrella potentials")
         before = self.exchanges_total()
         chains = self._chains(ladder_size)
@@ -202,7 +223,8 @@ class TemperingChains:
         ladder = int(ladder_size)
         if ladder > 1 and len(self.temperatures_host) % ladder:
             raise ValueError("ladder_size must divide replica count")
-        self._check_umbrella()
+        if ladder > 1 and self._umbrella():
+            raise ValueError("temperature-only exchange is disabled with umbrella potentials")
         chains = self._chains(ladder if ladder > 1 else None)
         _lib.check(_lib.lib().n3_temper_run(ctypes.byref(chains), int(sweeps), int(steps), ladder if ladder > 1 else 0,
                                             int(bad_bias), self.seed, self._stream()))
@@ -279,7 +301,9 @@ class TemperingChains:
 
 
 def main(argv=None) -> int:
-    """tempering.py:228-319 without umbrella sampling."""
+    """tempering.py:228-319, umbrella sampling included (--umbrella-max: one
+    center per ladder, linspace(0, max), strength --umbrella-strength, no
+    exchanges, and a frontier of the best state per distance bin of 8)."""
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--initial", nargs="+", required=True)
     parser.add_argument("--output", required=True)
@@ -295,8 +319,6 @@ def main(argv=None) -> int:
     parser.add_argument("--umbrella-max", type=float, default=0)
     parser.add_argument("--umbrella-strength", type=float, default=0.2)
     args = parser.parse_args(argv)
-    if args.umbrella_max:
-        raise NotImplementedError("umbrella potentials (--umbrella-max) are not supported by the device walk")
     output = Path(args.output)
     output.mkdir(parents=True, exist_ok=True)
     states, n = [], None
@@ -317,14 +339,21 @@ def main(argv=None) -> int:
                              dtype=np.int32)
         temperatures = np.tile(np.geomspace(args.tmin, args.tmax, args.ladder_size), args.replicas // args.ladder_size)
         chains = TemperingChains(columns, temperatures, args.seed)
+    if args.umbrella_max:
+        # tempering.py:265-267
+        centers = np.repeat(np.linspace(0, args.umbrella_max, args.replicas // args.ladder_size), args.ladder_size)
+        chains.centers = torch.from_numpy(centers.astype(np.float32)).to(chains.device)
+        chains.strengths = torch.full((args.replicas,), np.float32(args.umbrella_strength), dtype=torch.float32,
+                                      device=chains.device)
     reference = set((i // 2, int(c)) for i, c in enumerate(states[0]))
+    archive = {}
     (output / "parameters.json").write_text(json.dumps(vars(args), indent=2) + "\n", encoding="utf-8")
     start = time.perf_counter()
     best, proposals, exchanges = 10**9, 0, 0
     for epoch in range(1, args.rounds + 1):
         chains.advance(args.steps, args.bad_bias)
         proposals += args.steps * args.replicas
-        exchanges = chains.exchange(args.ladder_size)
+        exchanges = chains.exchange(args.ladder_size) if not args.umbrella_max else 0
         best_scores = chains.best_scores.cpu().numpy()
         index = int(np.argmin(best_scores))
         if int(best_scores[index]) < best:
@@ -346,10 +375,21 @@ def main(argv=None) -> int:
         if epoch % 10 == 0 or epoch == args.rounds:
             chains.validate([epoch % args.replicas])
             columns = chains.columns.cpu().numpy()
+            scores = chains.scores.cpu().numpy()
             distances = [len(reference.symmetric_difference((i // 2, int(c)) for i, c in enumerate(row)))
                          for row in columns]
+            if args.umbrella_max:
+                # tempering.py:296-303: best state per distance bin
+                for i, (score, distance) in enumerate(zip(scores, distances)):
+                    bucket = distance // 8
+                    if int(score) < archive.get(bucket, 10**9):
+                        archive[bucket] = int(score)
+                        candidate = chains.to_state(columns[i])
+                        write_candidate(output / f"frontier-bin{bucket:02d}.json", n, candidate.points(),
+                                        conflicts=int(score), symmetric_difference=distance, epoch=epoch,
+                                        seed=args.seed)
             payload = {"epoch": epoch, "proposals": proposals, "elapsed_seconds": time.perf_counter() - start,
-                       "best": best, "scores": chains.scores.cpu().numpy().tolist(), "distances": distances,
+                       "best": best, "scores": scores.tolist(), "distances": distances,
                        "exchanges": exchanges, "accepted": int(chains.accepted.cpu().numpy().astype(np.int64).sum())}
             with (output / "history.jsonl").open("a", encoding="utf-8") as stream:
                 stream.write(json.dumps(payload) + "\n")
@@ -358,7 +398,7 @@ def main(argv=None) -> int:
     chains.validate()
     chains.save(output / "checkpoint.npz")
     result = {"best": best, "proposals": proposals, "elapsed_seconds": time.perf_counter() - start,
-              "accepted": int(chains.accepted.cpu().numpy().astype(np.int64).sum()), "distance_frontier": {}}
+              "accepted": int(chains.accepted.cpu().numpy().astype(np.int64).sum()), "distance_frontier": archive}
     (output / "summary.json").write_text(json.dumps(result, indent=2) + "\n", encoding="utf-8")
     print(json.dumps(result), flush=True)
     return 0

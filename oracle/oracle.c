@@ -515,12 +515,49 @@ void n3o_exchange_thresholds(const float* temperatures, int replicas, int ladder
     }
 }
 
+/* floor(2^32 exp(-x)) for x > 0: the engine's umbrella acceptance word.
+ * Cody-Waite reduction -x = k ln2 + f with fdlibm's two-part ln2, then the
+ * degree-13 Taylor series of exp(f) by Horner's rule, every step an fma, a
+ * product, rint or ldexp, each rounded once under IEEE-754, so the device
+ * computes the same word.  Pinned against exp() in the tests (within one unit
+ * of the word). */
+__attribute__((optimize("fp-contract=off"))) uint32_t n3o_exp_word(double x)
+{
+    if (!(x < 23.0)) return 0u; /* 2^32 exp(-23) < 1 */
+    const double y = -x;
+    const double k = rint(y * 1.4426950408889634);
+    double f = fma(k, -6.93147180369123816490e-01, y);
+    f = fma(k, -1.90821492927058770002e-10, f);
+    static const double inv_fact[14] = {1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0,
+                                        1.0 / 362880.0,     1.0 / 40320.0,     1.0 / 5040.0,      1.0 / 720.0,
+                                        1.0 / 120.0,        1.0 / 24.0,        1.0 / 6.0,         0.5,
+                                        1.0,                1.0};
+    double p = inv_fact[0];
+    for (int i = 1; i < 14; ++i) p = fma(p, f, inv_fact[i]);
+    const double v = ldexp(p, (int)k + 32);
+    return v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
+}
+
+/* The umbrella energy of a move (tempering.py:93-94): delta + strength
+ * (2 (removed - center) dd + dd^2), with the products and sums in the
+ * order the device uses. */
+__attribute__((optimize("fp-contract=off"))) static double umbrella_energy(int delta, int dd, int removed,
+                                                                          float strength, float center)
+{
+    const double offset = (double)removed - (double)center;
+    const double a = fma(2.0 * offset, (double)dd, (double)(dd * dd));
+    return fma((double)strength, a, (double)delta);
+}
+
 /* One replica (slot) running `steps` moves of the walk kernel
- * (tempering.py:48-110) with umbrella strength 0.  Arrays are the slot's own
- * rows of the caller's arrays.  Returns the number of valid proposals. */
+ * (tempering.py:48-110).  Arrays are the slot's own rows of the caller's
+ * arrays; with `umbrella` the move energy carries the slot's umbrella term and
+ * is accepted against n3o_exp_word(E / T) (inverse_t = 1 / T).  Returns the
+ * number of valid proposals. */
 static int64_t walk_slot(int n, int steps, int bad_bias, uint64_t seed, uint32_t sweep, uint32_t slot,
                          int32_t* col, int32_t* bad, int32_t* score_io, int32_t* best_col, int32_t* best_score,
-                         int32_t* accepted, const int32_t* reference, int32_t* removed_io, const uint32_t* thr, int k)
+                         int32_t* accepted, const int32_t* reference, int32_t* removed_io, const uint32_t* thr, int k,
+                         int umbrella, float strength, float center, double inverse_t)
 {
     const int m = 2 * n;
     int32_t score = *score_io, removed = *removed_io;
@@ -586,10 +623,16 @@ static int64_t walk_slot(int n, int steps, int bad_bias, uint64_t seed, uint32_t
                 db[kk] += d;
             }
         }
-        int accept = delta <= 0;
-        if (!accept) {
-            const int idx = delta < k ? delta : k - 1;
-            accept = r[2] < thr[idx];
+        int accept;
+        if (umbrella) {
+            const double e = umbrella_energy(delta, dd, removed, strength, center);
+            accept = e <= 0.0 || r[2] < n3o_exp_word(e * inverse_t);
+        } else {
+            accept = delta <= 0;
+            if (!accept) {
+                const int idx = delta < k ? delta : k - 1;
+                accept = r[2] < thr[idx];
+            }
         }
         if (accept) {
             col[u] = cv;
@@ -609,10 +652,13 @@ static int64_t walk_slot(int n, int steps, int bad_bias, uint64_t seed, uint32_t
     return valid_count;
 }
 
-/* TemperingChains.advance (tempering.py:158-167) for every slot. */
+/* TemperingChains.advance (tempering.py:158-167) for every slot; `strengths`
+ * NULL means no umbrella potentials, else `centers` and `inverse_t` (1 / T per
+ * slot) are read too. */
 int64_t n3o_advance(int n, int replicas, int steps, int bad_bias, uint64_t seed, uint32_t sweep, int32_t* columns,
                     int32_t* badness, int32_t* scores, int32_t* best_columns, int32_t* best_scores, int32_t* accepted,
-                    const int32_t* reference, int32_t* removed, const uint32_t* thr, int k, int threads)
+                    const int32_t* reference, int32_t* removed, const uint32_t* thr, int k, int threads,
+                    const float* centers, const float* strengths, const double* inverse_t)
 {
     const int m = 2 * n;
     int64_t valid = 0;
@@ -623,7 +669,9 @@ int64_t n3o_advance(int n, int replicas, int steps, int bad_bias, uint64_t seed,
     for (int r = 0; r < replicas; ++r) {
         valid += walk_slot(n, steps, bad_bias, seed, sweep, (uint32_t)r, columns + (int64_t)r * m,
                            badness + (int64_t)r * m, scores + r, best_columns + (int64_t)r * m, best_scores + r,
-                           accepted + r, reference, removed + r, thr + (int64_t)r * k, k);
+                           accepted + r, reference, removed + r, thr + (int64_t)r * k, k, strengths != NULL,
+                           strengths ? strengths[r] : 0.0f, strengths ? centers[r] : 0.0f,
+                           strengths ? inverse_t[r] : 0.0);
     }
     return valid;
 }

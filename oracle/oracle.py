@@ -61,8 +61,11 @@ def lib():
         L.n3o_advance.argtypes = [
             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
             _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _u32p, ctypes.c_int, ctypes.c_int,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
         ]
         L.n3o_advance.restype = ctypes.c_int64
+        L.n3o_exp_word.argtypes = [ctypes.c_double]
+        L.n3o_exp_word.restype = ctypes.c_uint32
         L.n3o_exchange.argtypes = [
             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
             _i32p, _i32p, _i32p, _i32p, _u32p, _i8p, ctypes.c_int,
@@ -194,6 +197,12 @@ def delta_table(p_batch, q_batch) -> np.ndarray:
     return table
 
 
+def exp_word(x: float) -> int:
+    """floor(2^32 exp(-x)) as the engine's umbrella acceptance computes it
+    (n3o_exp_word)."""
+    return int(lib().n3o_exp_word(float(x)))
+
+
 def metropolis_thresholds(temperatures, k: int) -> np.ndarray:
     t = np.ascontiguousarray(temperatures, dtype=np.float32)
     thr = np.empty((len(t), k), dtype=np.uint32)
@@ -235,15 +244,26 @@ class TemperingReplay:
             [len(ref - set((i // 2, int(c)) for i, c in enumerate(row))) for row in self.columns], dtype=np.int32)
         self.k_move = k_move
         self.k_exchange = k_exchange
+        self.centers = np.zeros(self.replicas, dtype=np.float32)
+        self.strengths = np.zeros(self.replicas, dtype=np.float32)
         self.thr = metropolis_thresholds(self.temperatures, k_move)
         self.sweep = 0
         self.round = 0
 
     def advance(self, steps: int, bad_bias: int = 0, threads: int = 0) -> int:
+        """One sweep of `steps` proposals per slot, with the umbrella term of
+        tempering.py:92-95 when any of `strengths` is nonzero."""
+        umbrella = None
+        if np.any(self.strengths != 0):
+            centers = np.ascontiguousarray(self.centers, dtype=np.float32)
+            strengths = np.ascontiguousarray(self.strengths, dtype=np.float32)
+            inverse_t = 1.0 / self.temperatures.astype(np.float64)
+            umbrella = (centers, strengths, inverse_t)
+        ptrs = [None, None, None] if umbrella is None else [a.ctypes.data for a in umbrella]
         valid = lib().n3o_advance(
             self.n, self.replicas, steps, bad_bias, self.seed, self.sweep, self.columns, self.badness, self.scores,
             self.best_columns, self.best_scores, self.accepted, self.reference, self.removed_points, self.thr,
-            self.k_move, threads)
+            self.k_move, threads, *ptrs)
         self.sweep += 1
         return int(valid)
 
@@ -260,6 +280,8 @@ class TemperingReplay:
         return int(valid)
 
     def exchange(self, ladder_size: int) -> int:
+        if np.any(self.strengths != 0):
+            raise ValueError("temperature-only exchange is disabled with umbrella potentials")
         thr, sign = exchange_thresholds(self.temperatures, ladder_size, self.k_exchange)
         count = lib().n3o_exchange(self.n, self.replicas, ladder_size, self.seed, self.round, self.columns,
                                    self.badness, self.scores, self.removed_points, thr, sign, self.k_exchange)

@@ -41,6 +41,45 @@ def test_trajectory_matches_replay(pkg, oracle, dev, n, replicas, ladder, sweeps
     chains.validate()
 
 
+@pytest.mark.parametrize("n,replicas,ladder,sweeps,steps,bias", [
+    (12, 32, 8, 8, 24, 0),
+    (33, 64, 16, 5, 66, 30),
+    (75, 64, 16, 3, 150, 0),
+])
+def test_umbrella_trajectory_matches_replay(pkg, oracle, dev, n, replicas, ladder, sweeps, steps, bias):
+    """tempering.py:92-95 with the driver's umbrella set-up (one center per
+    ladder, linspace(0, 20), strength 0.2): every sweep bit-exact against the
+    replay, by advance() and by a graph run without exchanges."""
+    cols = _states(oracle, n, replicas, seed=n + 1)
+    temps = np.tile(np.geomspace(0.05, 2.0, ladder), replicas // ladder).astype(np.float32)
+    centers = np.repeat(np.linspace(0, 20, replicas // ladder), ladder).astype(np.float32)
+    seed = 777 + n
+    chains = pkg.TemperingChains(cols, temps, seed)
+    chains.centers = torch.from_numpy(centers).to(dev)
+    chains.strengths = torch.full((replicas,), 0.2, dtype=torch.float32, device=dev)
+    replay = oracle.TemperingReplay(cols, temps, seed, k_move=chains.k_move, k_exchange=2)
+    replay.centers[:] = centers
+    replay.strengths[:] = np.float32(0.2)
+    for _ in range(sweeps):
+        chains.advance(steps, bias)
+        replay.advance(steps, bias)
+        _compare(chains, replay)
+    graph = pkg.TemperingChains(cols, temps, seed)
+    graph.centers = torch.from_numpy(centers).to(dev)
+    graph.strengths = torch.full((replicas,), 0.2, dtype=torch.float32, device=dev)
+    graph.run(sweeps, steps, 0, bias)
+    torch.cuda.synchronize()
+    for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted", "removed_points"):
+        assert torch.equal(getattr(graph, name), getattr(chains, name)), name
+    chains.validate()
+    with pytest.raises(ValueError):
+        chains.exchange(ladder)
+    with pytest.raises(ValueError):
+        chains.run(1, steps, ladder)
+    with pytest.raises(NotImplementedError):
+        pkg.run_batched([pkg.BatchGroup(chains, ladder, steps)], 1)
+
+
 def test_graph_run_matches_step_by_step(pkg, oracle, dev):
     n, replicas, ladder = 20, 64, 16
     cols = _states(oracle, n, replicas, seed=5)
@@ -190,5 +229,21 @@ def test_driver_writes_the_reference_outputs(pkg, dev, tmp_path):
     assert pkg.score_population_gpu(np.array([again.P]), np.array([again.Q]))[0] == summary["best"]
     assert pkg.tempering.main(args[:-4] + ["--rounds", "10", "--steps", "30", "--resume",
                                            str(out / "checkpoint.npz")]) == 0
-    with pytest.raises(NotImplementedError):
-        pkg.tempering.main(args + ["--umbrella-max", "3"])
+    # umbrella sampling (tempering.py:265-267, 296-303): no exchanges, a
+    # frontier file per distance bin of 8
+    umb = tmp_path / "umbrella"
+    uargs = ["--initial", str(initial), "--output", str(umb), "--seed", "5", "--replicas", "32", "--ladder-size",
+             "8", "--rounds", "20", "--steps", "30", "--umbrella-max", "12"]
+    assert pkg.tempering.main(uargs) == 0
+    lines = [json.loads(x) for x in (umb / "history.jsonl").read_text().splitlines()]
+    assert all(x["exchanges"] == 0 for x in lines)
+    frontier = json.loads((umb / "summary.json").read_text())["distance_frontier"]
+    assert frontier and set(frontier) == {f.stem[len("frontier-bin"):].lstrip("0") or "0"
+                                          for f in umb.glob("frontier-bin*.json")}
+    for f in umb.glob("frontier-bin*.json"):
+        n, points, meta = pkg.io.load_candidate(f)
+        again = pkg.points_to_permutations(points, 14)
+        assert pkg.score_population_gpu(np.array([again.P]), np.array([again.Q]))[0] == meta["conflicts"]
+        assert meta["symmetric_difference"] // 8 == int(f.stem[len("frontier-bin"):])
+    saved = np.load(umb / "checkpoint.npz")
+    assert np.allclose(saved["strengths"], 0.2) and saved["centers"].max() == np.float32(12)

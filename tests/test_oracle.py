@@ -250,6 +250,55 @@ def test_replay_invariants(oracle):
     assert r.accepted.sum() > 0
 
 
+def test_exp_word_law(oracle):
+    """The umbrella acceptance word is floor(2^32 exp(-x)) to within one unit
+    (math.exp is not correctly rounded either), never grows with x, and is 0
+    from x = 23 on."""
+    import math
+
+    rng = np.random.default_rng(4)
+    xs = np.concatenate([np.linspace(1e-12, 22.99, 4001), rng.uniform(0, 23, 4000), [1e-300, 0.5, math.log(2)]])
+    for x in xs:
+        want = math.floor(math.ldexp(math.exp(-float(x)), 32))
+        got = oracle.exp_word(float(x))
+        assert abs(got - min(want, 2**32 - 1)) <= 1, x
+    words = [oracle.exp_word(float(x)) for x in np.sort(xs)]
+    assert all(a >= b for a, b in zip(words, words[1:]))
+    assert oracle.exp_word(23.0) == 0 and oracle.exp_word(40.0) == 0 and oracle.exp_word(1e-300) == 2**32 - 1
+
+
+def _umbrella_replay(oracle, centers_from, strength, sweeps=6):
+    n, replicas = 12, 16
+    p, q = oracle.generate_states(8, n, replicas)
+    cols = np.stack([np.column_stack((p[i], q[i])).reshape(-1) for i in range(replicas)]).astype(np.int32)
+    # the reference state is slot 0's; other slots start at some distance
+    temps = np.full(replicas, 0.5, np.float32)
+    r = oracle.TemperingReplay(cols, temps, 21, k_move=80, k_exchange=400)
+    r.strengths[:] = strength
+    r.centers[:] = centers_from(r.removed_points)
+    start = r.removed_points.copy()
+    for _ in range(sweeps):
+        r.advance(2 * n)
+        for i in range(replicas):
+            e, bad = oracle.score_and_badness(r.columns[i, 0::2], r.columns[i, 1::2])
+            assert e == r.scores[i] and np.array_equal(bad, r.badness[i])
+    return start, r
+
+
+def test_umbrella_energy_terms(oracle):
+    """tempering.py:93-94 on the replay.  A stiff umbrella centred on each
+    slot's own distance rejects every move that changes it (strength dd^2
+    dominates), while moves that keep it still happen; centred far above, the
+    -2 (center - removed) dd term accepts every move that raises it."""
+    start, r = _umbrella_replay(oracle, lambda removed: removed.astype(np.float32), 1e6)
+    assert np.array_equal(r.removed_points, start)
+    assert r.accepted.sum() > 0
+    start, r = _umbrella_replay(oracle, lambda removed: removed.astype(np.float32) + 1000.0, 1.0)
+    assert np.all(r.removed_points >= start) and r.removed_points.sum() > start.sum()
+    with pytest.raises(ValueError):
+        r.exchange(4)
+
+
 def test_metropolis_thresholds_law(oracle):
     thr = oracle.metropolis_thresholds(np.array([0.5, 2.0], np.float32), 64)
     for t, row in zip((0.5, 2.0), thr):

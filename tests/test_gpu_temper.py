@@ -80,6 +80,23 @@ def test_umbrella_trajectory_matches_replay(pkg, oracle, dev, n, replicas, ladde
         pkg.run_batched([pkg.BatchGroup(chains, ladder, steps)], 1)
 
 
+def test_config3_full_size_matches_replay(pkg, oracle, dev):
+    """Config 3 at full size (4,096 replicas as 64 ladders of 64
+    temperatures, 0.05..2.0, n=75, 150 proposals per sweep): three sweeps
+    with exchanges, every replica bit-exact against the replay."""
+    n, replicas, ladder, steps = 75, 4096, 64, 150
+    cols = _states(oracle, n, replicas, seed=31)
+    temps = np.tile(np.geomspace(0.05, 2.0, ladder), replicas // ladder).astype(np.float32)
+    chains = pkg.TemperingChains(cols, temps, 3)
+    replay = oracle.TemperingReplay(cols, temps, 3, k_move=chains.k_move,
+                                    k_exchange=chains._exchange_table(ladder)[2])
+    for _ in range(3):
+        chains.advance(steps)
+        replay.advance(steps, threads=oracle.cpu_count())
+        assert chains.exchange(ladder) == replay.exchange(ladder)
+        _compare(chains, replay)
+
+
 def test_graph_run_matches_step_by_step(pkg, oracle, dev):
     n, replicas, ladder = 20, 64, 16
     cols = _states(oracle, n, replicas, seed=5)

@@ -147,9 +147,9 @@ typedef struct n3_chains {
 
 /* `steps` proposals per replica (one sweep), TemperingChains.advance
  * (tempering.py:158-167, kernel walk tempering.py:29-115), with the umbrella
- * strength 0.  Proposal u, v and the acceptance word come from Philox keyed
- * by `seed` with counter (step, sweep, slot, "MOVE"); `sweep` is read from
- * counters[0], which the call then increments. */
+ * term when chains->strengths is set.  Proposal u, v and the acceptance word
+ * come from Philox keyed by `seed` with counter (step, sweep, slot, "MOVE");
+ * `sweep` is read from counters[0], which the call then increments. */
 int n3_temper_advance(const n3_chains* chains, int steps, int bad_bias, uint64_t seed, void* stream);
 
 /* Adjacent-temperature exchange inside ladders of `ladder` slots
@@ -161,7 +161,7 @@ int n3_temper_exchange(const n3_chains* chains, int ladder, uint64_t seed, void*
 /* `sweeps` x (advance(steps); exchange(ladder) if ladder > 1) replayed from a
  * captured hipGraph (the epoch loop of tempering.py:275-278).  The graph is
  * cached per (chains, steps, ladder, bad_bias, seed, stream) and rebuilt when
- * they change. */
+ * they change.  With umbrella strengths, ladder must be 0 or 1. */
 int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int ladder, int bad_bias, uint64_t seed,
                   void* stream);
 

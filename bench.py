@@ -228,6 +228,25 @@ def learned_ab_near_solution(pkg, seconds: float, moves: int = 12, seeds=(12, 13
             "learned_first": sum(1 for a, b in zip(t_u, t_l) if b is not None and (a is None or b < a))}
 
 
+def host_array_rates(pkg, torch, p, q, repeats: int = 5) -> dict:
+    """The drop-ins that take host NumPy arrays, timed with their PCIe copies
+    (pageable H2D of p, q = 39.3 MB; D2H of the scores, plus 39.3 MB of
+    badness for score_population_badness) on the same 65,536 states: the
+    PCIe-inclusive rate of the reference's boundary, not the headline."""
+    hp, hq = p.cpu().numpy(), q.cpu().numpy()
+    out = {}
+    for name in ("score_population_gpu", "score_population_badness"):
+        fn = getattr(pkg, name)
+        fn(hp, hq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            fn(hp, hq)
+        elapsed = (time.perf_counter() - t0) / repeats
+        out[name] = {"seconds_per_call": elapsed, "states_per_s": len(hp) / elapsed}
+    return out
+
+
 def search_rates(pkg) -> dict:
     """The device GA (population 65,536 children per generation, 256 elites)
     and the exhaustive 2-switch beam (32 elites) from one seeded n=75 state:
@@ -487,7 +506,7 @@ def main() -> int:
         roofline["counters"] = {k: prof.get(k) for k in ("valu_issue_fraction", "lds_active_fraction",
                                                           "lds_bank_conflict_fraction")}
 
-    extra = {}
+    extra = {"host_arrays": host_array_rates(pkg, torch, p, q)}
     legs = set() if args.no_temper else {x.strip() for x in args.legs.split(",") if x.strip()}
     unknown = legs - {"temper", "batched", "learned", "near", "search", "moves"}
     if unknown:

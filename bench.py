@@ -373,6 +373,12 @@ def tempering_cpu_baseline(pkg, oracle, seconds: float = 10.0) -> dict:
         replay.advance(steps, threads=cores)
         sweeps += 1
     elapsed = time.perf_counter() - t0
+    # one thread, as the reference's serial harnesses run: one sweep of the
+    # first 256 replicas
+    single = oracle.TemperingReplay(cols[:256], temps[:256], 3, k_move=k_move, k_exchange=2)
+    t1 = time.perf_counter()
+    single.advance(steps, threads=1)
+    single_rate = 256 * steps / (time.perf_counter() - t1)
     # the device walk over the same sweeps, states, temperatures and seed:
     # every replica's trajectory must match the replay's
     import torch
@@ -387,7 +393,7 @@ def tempering_cpu_baseline(pkg, oracle, seconds: float = 10.0) -> dict:
     return {"value": sweeps * steps * replicas / elapsed, "unit": "proposals/s", "cores": cores, "kind": "port",
             "sample": f"{sweeps} sweep(s) x {steps} proposals x {replicas} replicas (config 3's ladder), oracle C "
                       f"replay of tempering.walk with its pair-loop delta, OpenMP over replicas",
-            "trajectory_exact": exact}
+            "single_thread_proposals_per_s": single_rate, "trajectory_exact": exact}
 
 
 def move_cpu_baseline(pkg, oracle) -> dict:

@@ -185,9 +185,14 @@ class ProposalLog:
         self.count.zero_()
 
     def rows(self) -> torch.Tensor:
+        """The logged proposals in slot order (stable): a replica's entries
+        come in sweep and step order whatever order the replicas reserved
+        theirs in, so a log that holds every reservation yields the same rows,
+        in the same order, on every run of the same walk."""
         filled = min(int(self.count.item()), self.capacity)
         head = self.entries[:filled]
-        return head[head[:, 3] >= 0]
+        head = head[head[:, 3] >= 0]
+        return head[torch.argsort(head[:, 3], stable=True)]
 
 
 def training_tensors(entries: torch.Tensor, temperatures: np.ndarray, device, *, label: str = "metropolis",

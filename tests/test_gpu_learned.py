@@ -250,3 +250,11 @@ def test_trained_network_reaches_low_energy_in_fewer_sweeps(pkg, dev):
     assert s_learned is not None
     assert s_uniform is None or s_learned < s_uniform, (s_learned, s_uniform)
     learned.validate(range(0, 4096, 257))
+
+
+def test_pretraining_is_reproducible(pkg, dev):
+    """The same plain run logged and trained twice gives the same network:
+    the log is read in slot order and the gradient has no float atomics."""
+    a, _ = pkg.learned.pretrain(40, 1024, 64, seed=3, sweeps=20, log_sweeps=2, steps=60)
+    b, _ = pkg.learned.pretrain(40, 1024, 64, seed=3, sweeps=20, log_sweeps=2, steps=60)
+    assert torch.equal(a.master, b.master) and torch.equal(a.params, b.params)

@@ -37,6 +37,13 @@ int n3_version(void);
 /* Text of the last error recorded by this thread (never NULL). */
 const char* n3_last_error(void);
 
+/* Layout checks for bindings (no counterpart in the reference): the size in
+ * bytes of n3_chains (which = 0) or n3_group (which = 1), -1 otherwise; and
+ * the byte offsets of n3_chains' members in declaration order (writes at most
+ * `capacity`, returns the member count).  Host only, no device needed. */
+int64_t n3_struct_size(int which);
+int n3_chains_offsets(int64_t* offsets, int capacity);
+
 /* Checks that `device` is a gfx950 GPU and makes it current.  No reference
  * counterpart beyond gpu.py:50-51 ("no CUDA device is available"). */
 int n3_device_init(int device);

@@ -85,6 +85,8 @@ _u64 = ctypes.c_uint64
 SIGNATURES = {
     "n3_version": (_i, []),
     "n3_last_error": (ctypes.c_char_p, []),
+    "n3_struct_size": (_i64, [_i]),
+    "n3_chains_offsets": (_i, [_vp, _i]),
     "n3_device_init": (_i, [_i]),
     "n3_dirtable": (_i, [_i, _vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "n3_generate_states": (_i, [_u64, _i, _i64, _i64, _vp, _vp, _vp]),

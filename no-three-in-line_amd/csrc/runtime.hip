@@ -3,6 +3,7 @@
 #include "runtime.h"
 
 #include <atomic>
+#include <cstddef>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -162,6 +163,44 @@ using namespace no3;
 extern "C" int n3_version(void) { return 10000; }
 
 extern "C" const char* n3_last_error(void) { return g_error; }
+
+extern "C" int64_t n3_struct_size(int which) {
+  if (which == 0) return (int64_t)sizeof(n3_chains);
+  if (which == 1) return (int64_t)sizeof(n3_group);
+  return -1;
+}
+
+extern "C" int n3_chains_offsets(int64_t* offsets, int capacity) {
+  static const int64_t members[] = {
+      offsetof(n3_chains, n),
+      offsetof(n3_chains, replicas),
+      offsetof(n3_chains, columns),
+      offsetof(n3_chains, badness),
+      offsetof(n3_chains, scores),
+      offsetof(n3_chains, best_columns),
+      offsetof(n3_chains, best_scores),
+      offsetof(n3_chains, accepted),
+      offsetof(n3_chains, reference),
+      offsetof(n3_chains, removed_points),
+      offsetof(n3_chains, move_thresholds),
+      offsetof(n3_chains, k_move),
+      offsetof(n3_chains, exchange_thresholds),
+      offsetof(n3_chains, exchange_sign),
+      offsetof(n3_chains, k_exchange),
+      offsetof(n3_chains, counters),
+      offsetof(n3_chains, proposals),
+      offsetof(n3_chains, block_waves),
+      offsetof(n3_chains, log),
+      offsetof(n3_chains, log_capacity),
+      offsetof(n3_chains, log_count),
+      offsetof(n3_chains, centers),
+      offsetof(n3_chains, strengths),
+      offsetof(n3_chains, inverse_temperatures),
+  };
+  const int count = (int)(sizeof(members) / sizeof(members[0]));
+  for (int i = 0; offsets && i < count && i < capacity; ++i) offsets[i] = members[i];
+  return count;
+}
 
 extern "C" int n3_device_init(int device) {
   int count = 0;

@@ -24,6 +24,22 @@ def test_library_exports_header(pkg):
     assert lib.n3_version() >= 10000
 
 
+def test_struct_layouts_match_the_header(pkg):
+    """The ctypes mirrors of n3_chains and n3_group have the C layout."""
+    import ctypes
+
+    import numpy as np
+
+    lib = pkg._lib.lib()
+    assert lib.n3_struct_size(0) == ctypes.sizeof(pkg._lib.Chains)
+    assert lib.n3_struct_size(1) == ctypes.sizeof(pkg._lib.Group)
+    fields = [name for name, _ in pkg._lib.Chains._fields_]
+    out = np.zeros(64, dtype=np.int64)
+    count = lib.n3_chains_offsets(out.ctypes.data, len(out))
+    assert count == len(fields)
+    assert list(out[:count]) == [getattr(pkg._lib.Chains, f).offset for f in fields]
+
+
 def test_direction_table(pkg):
     import ctypes
 

@@ -162,6 +162,31 @@ def device(index: int | None = None) -> torch.device:
     return torch.device("cuda", idx)
 
 
+class Staging:
+    """A pinned host buffer and a device buffer of int32 words, reused by the
+    single-state calls of one device: one H2D of the packed inputs and one
+    D2H of the outputs per call instead of a pageable copy per array (such
+    calls are latency-bound).  Callers hold STAGING_LOCK while they use it."""
+
+    def __init__(self, dev: torch.device, words: int):
+        self.host = torch.empty(words, dtype=torch.int32, pin_memory=True)
+        self.host_np = self.host.numpy()
+        self.dev = torch.empty(words, dtype=torch.int32, device=dev)
+
+
+_STAGING: dict = {}
+STAGING_LOCK = threading.Lock()
+
+
+def staging(dev: torch.device, words: int) -> Staging:
+    """The device's staging buffers, grown to at least `words` words."""
+    st = _STAGING.get(dev.index)
+    if st is None or st.host.numel() < words:
+        st = Staging(dev, max(words, 1024))
+        _STAGING[dev.index] = st
+    return st
+
+
 def stream_handle(dev: torch.device) -> int:
     return torch.cuda.current_stream(dev).cuda_stream
 

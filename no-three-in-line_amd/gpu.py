@@ -100,6 +100,21 @@ def gpu_device_name() -> str:
 
 
 def score_and_badness(p, q) -> tuple[int, np.ndarray]:
-    """accelerated.py:14-37 on the GPU for one state: (score, badness int64[2n])."""
-    scores, badness = score_population_badness(np.asarray(p)[None, :], np.asarray(q)[None, :])
-    return int(scores[0]), badness[0].astype(np.int64)
+    """accelerated.py:14-37 on the GPU for one state: (score, badness int64[2n]).
+    The state goes through the device's pinned staging buffer: one copy in,
+    one copy out."""
+    p_host, q_host = _validate_batches(np.asarray(p)[None, :], np.asarray(q)[None, :])
+    n = p_host.shape[1]
+    dev = _lib.device()
+    with _lib.STAGING_LOCK:
+        st = _lib.staging(dev, 4 * n + 1)
+        h = st.host_np
+        h[:n], h[n:2 * n] = p_host[0], q_host[0]
+        st.dev[:2 * n].copy_(st.host[:2 * n], non_blocking=True)
+        base = st.dev.data_ptr()
+        stream = _lib.stream_handle(dev)
+        _lib.check(_lib.lib().n3_score_population(base, base + 4 * n, n, 1, base + 16 * n, base + 8 * n,
+                                                  _lib.SCORE_AUTO, stream))
+        st.host[2 * n:4 * n + 1].copy_(st.dev[2 * n:4 * n + 1], non_blocking=True)
+        _lib.check(_lib.lib().n3_score_status(stream))  # synchronises the stream
+        return int(h[4 * n]), h[2 * n:4 * n].astype(np.int64)

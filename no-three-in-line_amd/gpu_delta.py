@@ -7,8 +7,6 @@ produces the full m x m table of a population.
 """
 from __future__ import annotations
 
-import threading
-
 import numpy as np
 import torch
 
@@ -24,37 +22,13 @@ def _state_arrays(p, q, badness, dtype):
     return p_host, q_host, bad_host
 
 
-class _Staging:
-    """A pinned host buffer and a device buffer of int32 words, reused by the
-    single-state calls of one device: one H2D of the packed inputs and one
-    D2H of the scores per call instead of three pageable copies in and one
-    out (the call is latency-bound)."""
-
-    def __init__(self, dev: torch.device, words: int):
-        self.host = torch.empty(words, dtype=torch.int32, pin_memory=True)
-        self.host_np = self.host.numpy()
-        self.dev = torch.empty(words, dtype=torch.int32, device=dev)
-
-
-_STAGING: dict[int, _Staging] = {}
-_STAGING_LOCK = threading.Lock()
-
-
-def _staging(dev: torch.device, words: int) -> _Staging:
-    st = _STAGING.get(dev.index)
-    if st is None or st.host.numel() < words:
-        st = _Staging(dev, max(words, 1024))
-        _STAGING[dev.index] = st
-    return st
-
-
 def score_two_switches_gpu(p, q, selected_index: int, current_score: int, badness) -> np.ndarray:
     """gpu_delta.py:86-116: scores after switching slot `selected_index` with every slot."""
     p_host, q_host, bad_host = _state_arrays(p, q, badness, np.int32)
     n = len(p_host)
     dev = _lib.device()
-    with _STAGING_LOCK:
-        st = _staging(dev, 6 * n)
+    with _lib.STAGING_LOCK:
+        st = _lib.staging(dev, 6 * n)
         h = st.host_np
         h[:n], h[n:2 * n], h[2 * n:4 * n] = p_host, q_host, bad_host
         st.dev[:4 * n].copy_(st.host[:4 * n], non_blocking=True)

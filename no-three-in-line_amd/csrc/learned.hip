@@ -258,7 +258,11 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     const uint64_t s1sq = (uint64_t)s1 * s1;
     const uint64_t big_n = s1sq + mm4 * wu * wv;  // < 2^48 (w <= 2^15, m <= 200)
     // badness of every slot in the moved state (needed for the reverse move)
-    int cnt_u2[CH], cnt_v2[CH], cnt_u[CH], cnt_v[CH], id[CH];
+    int cnt_u2[CH], cnt_v2[CH], cnt_u[CH], cnt_v[CH], idv[CH];
+    // the old state's directions from v, looked up here so that the reads
+    // overlap the first histograms instead of waiting behind the last one
+#pragma unroll
+    for (int c = 0; c < CH; ++c) idv[c] = table[key[c] - key_v + offset];
     bool include[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
@@ -267,7 +271,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
     direction_counts_known<CH>(hist, trash, idv2, include, cnt_v2);
-    direction_counts<CH>(table, hist, trash, offset, key_v, key, include, cnt_v, id);
+    direction_counts_known<CH>(hist, trash, idv, include, cnt_v);
     int nb[CH], w2[CH];
     uint32_t part2 = 0;
 #pragma unroll

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
-        old_u2[c] = atomicAdd(&hist[word_u2[c]], 1u << ((uint32_t)idu2[c] << 3));
+        old_u2[c] = atomicAdd(&hist[word_u2[c]], byte_one((uint32_t)idu2[c]));
       }
       wave_sync();
 #pragma unroll
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
-        old_v2[c] = atomicAdd(&hist[word_v2[c]], 1u << ((uint32_t)idv2[c] << 3));
+        old_v2[c] = atomicAdd(&hist[word_v2[c]], byte_one((uint32_t)idv2[c]));
       }
       wave_sync();
 #pragma unroll
@@ -156,8 +156,8 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int k = lane + kWave * c;
-        part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
-                __builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8);
+        part += __builtin_amdgcn_ubfe(old_u2[c], byte_shift((uint32_t)idu2[c]), 8) +
+                __builtin_amdgcn_ubfe(old_v2[c], byte_shift((uint32_t)idv2[c]), 8);
         joint_new += __popcll(lanes((k < m) && k != u && k != v) & lanes(idu2[c] == id_u2v2));
       }
       // bad'(u') + bad'(v') (each below 2^16, so the sum fits one word)
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(256) three_cycle_kernel(const int32_t* __restr
       for (int c = 0; c < CH; ++c) {
         const int k = lane + kWave * c;
         word[c] = (k < m && k != anchor_slot) ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
-        old[c] = atomicAdd(&hist[word[c]], 1u << ((uint32_t)id[c] << 3));
+        old[c] = atomicAdd(&hist[word[c]], byte_one((uint32_t)id[c]));
       }
       wave_sync();
 #pragma unroll
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(256) three_cycle_kernel(const int32_t* __restr
     uint32_t part_uv = 0, part_w = 0;
     int joint_new = 0;
     auto rank_of = [&](uint32_t old, int id, int k, int anchor_slot) -> uint32_t {
-      return (k < m && k != anchor_slot) ? __builtin_amdgcn_ubfe(old, (uint32_t)id << 3, 8) : 0u;
+      return (k < m && k != anchor_slot) ? __builtin_amdgcn_ubfe(old, byte_shift((uint32_t)id), 8) : 0u;
     };
 #pragma unroll
     for (int c = 0; c < CH; ++c) {

@@ -218,7 +218,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
-      old_u2[c] = atomicAdd(&hist[word_u2[c]], 1u << ((uint32_t)idu2[c] << 3));
+      old_u2[c] = atomicAdd(&hist[word_u2[c]], byte_one((uint32_t)idu2[c]));
     }
     wave_sync();
 #pragma unroll
@@ -227,7 +227,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
-      old_v2[c] = atomicAdd(&hist[word_v2[c]], 1u << ((uint32_t)idv2[c] << 3));
+      old_v2[c] = atomicAdd(&hist[word_v2[c]], byte_one((uint32_t)idv2[c]));
     }
     wave_sync();
 #pragma unroll
@@ -239,8 +239,8 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     int joint_new = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
-              (__builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16);
+      part += __builtin_amdgcn_ubfe(old_u2[c], byte_shift((uint32_t)idu2[c]), 8) +
+              (__builtin_amdgcn_ubfe(old_v2[c], byte_shift((uint32_t)idv2[c]), 8) << 16);
       joint_new += __popcll(lanes(others[c]) & lanes(idu2[c] == id_u2v2));
     }
     const uint32_t packed = wave_sum_dpp(part);

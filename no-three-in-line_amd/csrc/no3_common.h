@@ -49,6 +49,12 @@ __host__ __device__ inline int trash_word(const DirTableInfo& info, int lane) {
   return info.hist_words - kWave + lane;
 }
 
+// Byte-packed histograms keep direction id's 8-bit counter in word id >> 2,
+// byte id & 3: its bit offset, and the word increment that adds one to it.
+// The offset is masked so that no shift count reaches 32.
+__host__ __device__ inline uint32_t byte_shift(uint32_t id) { return (id & 3u) << 3; }
+__host__ __device__ inline uint32_t byte_one(uint32_t id) { return 1u << byte_shift(id); }
+
 __host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
   return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
 }

@@ -275,9 +275,7 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
     // among the earlier points of its direction); the anchor's own lane hits
     // the zero-vector bin alone and adds 0
     // One anchor: histogram build with returned ranks, then clear.  Only the
-    // last chunk can hold lanes past the last slot (m > 64 (CH-1)).  The
-    // shift (id << 3) is used unmasked: the shifts and bit-field extracts
-    // read its low five bits.
+    // last chunk can hold lanes past the last slot (m > 64 (CH-1)).
     auto anchor_part = [&](int anchor_key) -> uint32_t {
       const int base = info.offset - anchor_key;
       constexpr int L = CH - 1;  // the only chunk that can hold idle lanes
@@ -288,9 +286,9 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
       id[L] = 0;
       if (tail) id[L] = table[key[L] + base];
 #pragma unroll
-      for (int c = 0; c < L; ++c) old[c] = atomicAdd(&hist[id[c] >> 2], 1u << (id[c] << 3));
+      for (int c = 0; c < L; ++c) old[c] = atomicAdd(&hist[id[c] >> 2], byte_one(id[c]));
       old[L] = 0;
-      if (tail) old[L] = atomicAdd(&hist[id[L] >> 2], 1u << (id[L] << 3));
+      if (tail) old[L] = atomicAdd(&hist[id[L] >> 2], byte_one(id[L]));
       wave_sync();
 #pragma unroll
       for (int c = 0; c < L; ++c) hist[id[c] >> 2] = 0u;
@@ -298,7 +296,7 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
       wave_sync();
       uint32_t part = 0;
 #pragma unroll
-      for (int c = 0; c < CH; ++c) part += __builtin_amdgcn_ubfe(old[c], id[c] << 3, 8);
+      for (int c = 0; c < CH; ++c) part += __builtin_amdgcn_ubfe(old[c], byte_shift(id[c]), 8);
       return part;
     };
     int bad[CH];

@@ -242,7 +242,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
-      old_u2[c] = atomicAdd(&hist[word_u2[c]], 1u << ((uint32_t)idu2[c] << 3));
+      old_u2[c] = atomicAdd(&hist[word_u2[c]], byte_one((uint32_t)idu2[c]));
     }
     wave_sync();
 #pragma unroll
@@ -251,7 +251,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
-      old_v2[c] = atomicAdd(&hist[word_v2[c]], 1u << ((uint32_t)idv2[c] << 3));
+      old_v2[c] = atomicAdd(&hist[word_v2[c]], byte_one((uint32_t)idv2[c]));
     }
     wave_sync();
 #pragma unroll
@@ -266,8 +266,8 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     for (int c = 0; c < CH; ++c) {
       // trash words (one per lane, last chunk only) and the zero vector of
       // the moved slot return rank 0
-      part += __builtin_amdgcn_ubfe(old_u2[c], (uint32_t)idu2[c] << 3, 8) +
-              (__builtin_amdgcn_ubfe(old_v2[c], (uint32_t)idv2[c] << 3, 8) << 16);
+      part += __builtin_amdgcn_ubfe(old_u2[c], byte_shift((uint32_t)idu2[c]), 8) +
+              (__builtin_amdgcn_ubfe(old_v2[c], byte_shift((uint32_t)idv2[c]), 8) << 16);
       joint_new += __popcll(lanes(others[c]) & lanes(idu2[c] == id_u2v2));
     }
     // bad'(u') in the low half, bad'(v') in the high half (each < 2^16)

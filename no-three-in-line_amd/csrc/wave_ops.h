@@ -86,11 +86,11 @@ __device__ inline void direction_counts(const uint16_t* table, uint32_t* hist, i
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     word[c] = include[c] ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
-    atomicAdd(&hist[word[c]], 1u << ((uint32_t)id[c] << 3));
+    atomicAdd(&hist[word[c]], byte_one((uint32_t)id[c]));
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], (uint32_t)id[c] << 3, 8);
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], byte_shift((uint32_t)id[c]), 8);
   wave_sync();
 #pragma unroll
   for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
@@ -105,11 +105,11 @@ __device__ inline void direction_counts_known(uint32_t* hist, int trash, const i
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     word[c] = include[c] ? ((uint32_t)id[c] >> 2) : (uint32_t)trash;
-    atomicAdd(&hist[word[c]], 1u << ((uint32_t)id[c] << 3));
+    atomicAdd(&hist[word[c]], byte_one((uint32_t)id[c]));
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], (uint32_t)id[c] << 3, 8);
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], byte_shift((uint32_t)id[c]), 8);
   wave_sync();
 #pragma unroll
   for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;

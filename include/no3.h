@@ -221,6 +221,19 @@ int n3_mlp_forward(const uint16_t* params, const uint16_t* x, int64_t rows, floa
 int n3_mlp_grad(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y, int64_t pairs,
                 float* grad, float* loss, void* stream);
 
+/* n3_mlp_grad without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight training loops).  The per-block partial sums live in a
+ * file-scope device array of the library (641 KiB per GPU, held until the
+ * process ends; its address is looked up by n3_device_init or by the first
+ * call on a device) instead of a buffer the library allocates.  The scratch
+ * holds the partials of at most 128 blocks and the gradient grid never
+ * exceeds 128 blocks (they stride over the tuples), so every pairs >= 1
+ * fits.  Same result bits as n3_mlp_grad.  All calls on one GPU share the
+ * array: calls that overlap on one GPU race. */
+int n3_mlp_grad_noalloc(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y,
+                        int64_t pairs, float* grad, float* loss, void* stream);
+
 /* One Adam step on the fp32 master copy; writes the bf16 parameters. */
 int n3_mlp_adam(float* master, uint16_t* params, const float* grad, float* m, float* v, int step, float lr,
                 float beta1, float beta2, float eps, void* stream);

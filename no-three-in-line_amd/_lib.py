@@ -110,6 +110,7 @@ SIGNATURES = {
     "n3_mlp_features": (_i, [_i, ctypes.c_float, _i, _vp]),
     "n3_mlp_forward": (_i, [_vp, _vp, _i64, _vp, _vp]),
     "n3_mlp_grad": (_i, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "n3_mlp_grad_noalloc": (_i, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "n3_mlp_adam": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                          ctypes.c_float, _vp]),
     "n3_mlp_weight_table": (_i, [_vp, _vp, _i, _i, _vp, _vp]),

@@ -9,6 +9,8 @@
 // ends up holding one row's pre-activations for 16 hidden units per tile and
 // the output layer is a per-lane dot product plus one cross-half exchange.
 // The weight gradient dW1 = dH X is a second MFMA product with the rows as K.
+#include <mutex>
+
 #include "runtime.h"
 #include "wave_ops.h"
 
@@ -323,6 +325,54 @@ static float* grad_partials(int* st_out) {
   return buffers[device];
 }
 
+// Both gradient entry points: the partial sums of at most kGradMaxBlocks
+// blocks (their grid strides over any number of tiles), then the ordered
+// reduction into grad and loss.  Every column the reduction reads is written
+// by this call's gradient kernel first, so the buffer needs no clearing.
+static int launch_grad(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y,
+                       int64_t pairs, float* grad, float* loss, float* partial, hipStream_t s) {
+  const long long tiles = (pairs + 31) / 32;
+  long long blocks = (tiles + kGradWaves - 1) / kGradWaves;
+  if (blocks > kGradMaxBlocks) blocks = kGradMaxBlocks;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3((unsigned)blocks), dim3(kGradWaves * kWave), 0, s, params, xu, xv, y,
+                     (long long)pairs, partial);
+  int st = hip_status(hipGetLastError(), "mlp_grad_kernel launch");
+  if (st) return st;
+  const int reduce_waves = 256 / kWave;
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3((kPartial + reduce_waves - 1) / reduce_waves), dim3(256), 0, s, partial,
+                     (int)blocks, grad, loss);
+  return hip_status(hipGetLastError(), "grad_reduce_kernel launch");
+}
+
+// The partial-sum buffer of n3_mlp_grad_noalloc, for callers that cannot
+// allocate: a file-scope device array of kPartial x kGradMaxBlocks floats
+// (641 KiB), which every GPU the library runs on holds until the process
+// ends.  Each GPU has its own copy at its own address; in host code the name
+// `scratch` is only a stand-in, so the host passes the address that
+// hipGetSymbolAddress reports for the current device (kept per device) and
+// never the name itself.  All calls on one GPU share that copy.
+static __device__ __align__(256) float scratch[kPartial * kGradMaxBlocks];
+
+static std::mutex scratch_mutex;
+static float* scratch_address[64] = {nullptr};
+
+// Looks up (once per device, with `device` current) and returns the device
+// address of `scratch`.  Called by n3_device_init, and by the first
+// n3_mlp_grad_noalloc on a device that was not initialised that way.
+int mlp_scratch_setup(int device, float** address) {
+  if (device < 0 || device >= 64) return set_error(N3_ENODEV, "device index %d out of range", device);
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[device]) {
+    void* ptr = nullptr;
+    const int st = hip_status(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (st) return st;
+    if (!ptr) return set_error(N3_ENODEV, "no device address for the gradient scratch on device %d", device);
+    scratch_address[device] = static_cast<float*>(ptr);
+  }
+  if (address) *address = scratch_address[device];
+  return N3_OK;
+}
+
 __global__ void adam_kernel(float* __restrict__ master, uint16_t* __restrict__ params, const float* __restrict__ grad,
                             float* __restrict__ m, float* __restrict__ v, int step, float lr, float beta1, float beta2,
                             float eps) {
@@ -407,18 +457,19 @@ extern "C" int n3_mlp_grad(const uint16_t* params, const uint16_t* xu, const uin
   int st = N3_OK;
   float* partial = grad_partials(&st);
   if (st) return st;
-  const long long tiles = (pairs + 31) / 32;
-  long long blocks = (tiles + kGradWaves - 1) / kGradWaves;
-  if (blocks > kGradMaxBlocks) blocks = kGradMaxBlocks;
-  hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3((unsigned)blocks), dim3(kGradWaves * kWave), 0, s, params, xu, xv, y,
-                     (long long)pairs, partial);
-  st = hip_status(hipGetLastError(), "mlp_grad_kernel launch");
+  return launch_grad(params, xu, xv, y, pairs, grad, loss, partial, as_stream(stream));
+}
+
+extern "C" int n3_mlp_grad_noalloc(const uint16_t* params, const uint16_t* xu, const uint16_t* xv, const float* y,
+                                   int64_t pairs, float* grad, float* loss, void* stream) {
+  if (!params || !xu || !xv || !y || !grad || !loss || pairs < 1) return set_error(N3_EINVAL, "bad grad arguments");
+  int device = 0;
+  int st = hip_status(hipGetDevice(&device), "hipGetDevice");
   if (st) return st;
-  const int reduce_waves = 256 / kWave;
-  hipLaunchKernelGGL(grad_reduce_kernel, dim3((kPartial + reduce_waves - 1) / reduce_waves), dim3(256), 0, s, partial,
-                     (int)blocks, grad, loss);
-  return hip_status(hipGetLastError(), "grad_reduce_kernel launch");
+  float* partial = nullptr;
+  st = mlp_scratch_setup(device, &partial);
+  if (st) return st;
+  return launch_grad(params, xu, xv, y, pairs, grad, loss, partial, as_stream(stream));
 }
 
 extern "C" int n3_mlp_adam(float* master, uint16_t* params, const float* grad, float* m, float* v, int step, float lr,

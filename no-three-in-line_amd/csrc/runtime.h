@@ -18,6 +18,10 @@ int check_columns_launch_args(int n, int64_t count);
 // device, queried once per device.
 int device_limits(int* lds_limit, int* compute_units);
 
+// Device address of the no-allocation gradient's scratch array on `device`
+// (current), looked up once per device (csrc/mlp.hip).
+int mlp_scratch_setup(int device, float** address);
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 // LDS bytes of a direction table rounded up to 16 bytes.

@@ -212,7 +212,9 @@ extern "C" int n3_device_init(int device) {
   if (st) return st;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return set_error(N3_ENODEV, "device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
-  return hip_status(hipSetDevice(device), "hipSetDevice");
+  st = hip_status(hipSetDevice(device), "hipSetDevice");
+  if (st) return st;
+  return mlp_scratch_setup(device, nullptr);
 }
 
 extern "C" int n3_dirtable(int n, uint16_t* host_table, int* directions, int* hist_words) {

@@ -748,6 +748,14 @@ void n3o_metropolis_thresholds64(const float* temperatures, int replicas, int k,
  *   r3 * N * S1'^2 < thr64(delta) * N' * S1^2,  N = S1^2 + 4 m^2 w(u) w(v),
  * evaluated exactly (primes for the moved state).  Philox counter
  * (step, sweep, slot, "LRND"). */
+/* Planted mistakes in the learned walk's Metropolis-Hastings rule, for the
+ * tests that show a detailed-balance check catches them (0 = the rule as
+ * specified; 1 = S1 and S1' swapped in the acceptance test; 2 = the uniform
+ * term dropped from N and N'; 3 = the moved state's weights read at the old
+ * energy).  Test infrastructure only. */
+static int learned_variant = 0;
+void n3o_set_learned_variant(int variant) { learned_variant = variant; }
+
 /* Learned walk weights are clamped to [1, 2^15]. */
 static inline uint32_t clamp_weight(uint32_t w) { return w < 1u ? 1u : (w > 32768u ? 32768u : w); }
 
@@ -853,17 +861,25 @@ static int64_t walk_slot_learned(int n, int steps, uint64_t seed, uint32_t sweep
         uint32_t s1n = 0;
         for (int i = 0; i < m; ++i) {
             nb[i] = bad[i] + db[i];
-            w2[i] = clamp_weight(wt[ROW(score + delta) + (nb[i] < 63 ? nb[i] : 63)]);
+            const int32_t e2 = learned_variant == 3 ? score : score + delta;
+            w2[i] = clamp_weight(wt[ROW(e2) + (nb[i] < 63 ? nb[i] : 63)]);
             s1n += w2[i];
         }
         /* pair probability 2 (0.2/m^2 + 0.8 w(u) w(v)/S1^2) = 0.4 N / (m^2 S1^2);
          * accept iff r N S1'^2 < thr64(delta) N' S1^2 */
         const uint64_t mm4 = 4ull * (uint64_t)m * (uint64_t)m;
-        const uint64_t big_n = (uint64_t)s1 * s1 + mm4 * w[u] * w[v];
-        const uint64_t big_n2 = (uint64_t)s1n * s1n + mm4 * w2[u] * w2[v];
+        const uint64_t uniform_sq = learned_variant == 2 ? 0 : 1;
+        const uint64_t big_n = uniform_sq * s1 * s1 + mm4 * w[u] * w[v];
+        const uint64_t big_n2 = uniform_sq * s1n * s1n + mm4 * w2[u] * w2[v];
         const int d = delta < -k64 ? -k64 : (delta > k64 - 1 ? k64 - 1 : delta);
-        const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * ((uint64_t)s1n * s1n);
-        if (!below_product(lhs, (unsigned __int128)thr64[d + k64] * big_n2, (uint64_t)s1 * s1)) continue;
+        uint64_t old_sq = (uint64_t)s1 * s1, new_sq = (uint64_t)s1n * s1n;
+        if (learned_variant == 1) {
+            const uint64_t t = old_sq;
+            old_sq = new_sq;
+            new_sq = t;
+        }
+        const unsigned __int128 lhs = (unsigned __int128)r[3] * big_n * new_sq;
+        if (!below_product(lhs, (unsigned __int128)thr64[d + k64] * big_n2, old_sq)) continue;
         col[u] = cv;
         col[v] = cu;
         score += delta;

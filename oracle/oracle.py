@@ -79,6 +79,8 @@ def lib():
             ctypes.c_int, ctypes.c_int,
         ]
         L.n3o_advance_learned.restype = ctypes.c_int64
+        L.n3o_set_learned_variant.argtypes = [ctypes.c_int]
+        L.n3o_set_learned_variant.restype = None
         _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
         L.n3o_mutate_population.argtypes = [
             _i32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
@@ -287,6 +289,12 @@ class TemperingReplay:
                                    self.badness, self.scores, self.removed_points, thr, sign, self.k_exchange)
         self.round += 1
         return int(count)
+
+
+def set_learned_variant(variant: int) -> None:
+    """Plants a mistake in the oracle's learned walk (0 = none; see
+    n3o_set_learned_variant), for tests that show their check catches it."""
+    lib().n3o_set_learned_variant(int(variant))
 
 
 def cpu_count() -> int:

@@ -112,6 +112,39 @@ def test_points_to_permutations(pkg):
     assert pkg.points_to_permutations(s.points(), 6).points() == s.points()
 
 
+def _components(n, points):
+    """Connected components of the row/column graph of a point set."""
+    parent = list(range(2 * n))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for r, c in points:
+        parent[find(r)] = find(n + c)
+    return len({find(a) for a in range(2 * n)})
+
+
+def test_points_to_permutations_returns_the_reference_colouring(pkg):
+    """The colouring (which of the 2^(c-1) choices) equals the reference's
+    on 141 random 2-factors, n = 5..75 with 1 to 5+ components
+    (tests/golden/points_to_permutations.json, written by the reference's
+    state.points_to_permutations)."""
+    import json
+
+    data = json.loads((ROOT / "tests" / "golden" / "points_to_permutations.json").read_text())
+    multi = 0
+    for case in data["cases"]:
+        n = case["n"]
+        points = [tuple(p) for p in case["points"]]
+        state = pkg.points_to_permutations(points, n)
+        assert list(state.P) == case["P"] and list(state.Q) == case["Q"], n
+        multi += _components(n, points) > 1
+    assert len(data["cases"]) == 141 and multi >= 90
+
+
 def test_three_cycle_candidates_match_reference_loop(pkg):
     import importlib
 

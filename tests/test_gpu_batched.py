@@ -36,6 +36,66 @@ def test_batched_equals_separate_runs(pkg, dev):
         assert torch.equal(g.chains.columns, b.columns) and torch.equal(g.chains.scores, b.scores)
 
 
+CONFIG4 = {40: 8192, 52: 4096, 61: 4096, 75: 2048, 83: 2048, 100: 1024}
+
+
+def _columns(chains_or_replay):
+    c = chains_or_replay.columns
+    return c.cpu().numpy() if hasattr(c, "cpu") else c
+
+
+def test_config4_sizes_match_the_replay(pkg, oracle, dev):
+    """Config 4's six sizes in ONE batched run (64 replicas each, ladders of
+    16 temperatures 0.05..2.0, 2n proposals per sweep, exchanges every
+    sweep): every group's replicas bit-exact against the CPU replay of the
+    reference walk after every run of sweeps."""
+    groups, replays = [], []
+    for i, n in enumerate(CONFIG4):
+        replicas, ladder = 64, 16
+        chains = pkg.seeded_chains(n, replicas, ladder, seed=500 + i)
+        cols = chains.columns.cpu().numpy()
+        temps = chains.temperatures_host
+        replay = oracle.TemperingReplay(cols, temps, chains.seed, k_move=chains.k_move,
+                                        k_exchange=chains._exchange_table(ladder)[2])
+        groups.append(pkg.BatchGroup(chains, ladder, 2 * n))
+        replays.append((replay, ladder, 2 * n))
+    for sweeps in (2, 1):
+        print(f"config-4 sizes vs replay: {sweeps} sweep(s)", flush=True)
+        pkg.run_batched(groups, sweeps)
+        torch.cuda.synchronize()
+        for g, (replay, ladder, steps) in zip(groups, replays):
+            for _ in range(sweeps):
+                replay.advance(steps, threads=oracle.cpu_count())
+                replay.exchange(ladder)
+            for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted",
+                         "removed_points"):
+                got = getattr(g.chains, name).cpu().numpy()
+                assert np.array_equal(got, getattr(replay, name)), (g.chains.n, name)
+
+
+def test_config4_full_counts(pkg, dev):
+    """Config 4 at its full replica counts (21,504 replicas over six sizes,
+    ladders of 64, 2n proposals per sweep, tuned-by-hand shapes): a few
+    batched sweeps, then a determinant recount of a sample of every group."""
+    shapes = {40: 16, 52: 16, 61: 16, 75: 8, 83: 8, 100: 4}
+    groups = []
+    for i, (n, replicas) in enumerate(CONFIG4.items()):
+        chains = pkg.seeded_chains(n, replicas, 64, seed=900 + i)
+        chains.block_waves = shapes[n]
+        groups.append(pkg.BatchGroup(chains, 64, 2 * n))
+    start = [g.chains.scores.clone() for g in groups]
+    pkg.run_batched(groups, 6)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(0)
+    for g, s0 in zip(groups, start):
+        chains = g.chains
+        total = chains.columns.shape[0]
+        chains.validate(rng.choice(total, size=min(total, 256), replace=False))
+        assert int(chains.counters[0]) == 6 and int(chains.counters[1]) == 6
+        assert int(chains.proposals[0]) > 0 and int(chains.accepted.sum()) > 0
+        assert float(chains.scores.float().mean()) < float(s0.float().mean())
+
+
 def test_batched_argument_errors(pkg, dev):
     a = pkg.seeded_chains(12, 32, 16, seed=1)
     with pytest.raises(ValueError):

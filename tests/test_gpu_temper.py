@@ -23,6 +23,10 @@ def _compare(chains, replay):
     (33, 64, 32, 6, 66, 0),
     (75, 64, 64, 4, 150, 0),
     (75, 64, 16, 3, 150, 50),
+    # three chunks of slots past n=64, four chunks from n=97 on
+    (83, 64, 32, 3, 166, 0),
+    (97, 32, 16, 2, 194, 0),
+    (100, 32, 32, 2, 200, 30),
 ])
 def test_trajectory_matches_replay(pkg, oracle, dev, n, replicas, ladder, sweeps, steps, bias):
     cols = _states(oracle, n, replicas, seed=n)

@@ -92,10 +92,40 @@ def candidate_scores(elite_p: torch.Tensor, elite_q: torch.Tensor, elite_scores:
 
 
 def _signature(p: np.ndarray, q: np.ndarray) -> bytes:
-    """gpu_exhaustive_beam.py:85-88."""
-    values = np.column_stack((p, q))
-    values.sort(axis=1)
-    return values.astype(np.int16, copy=False).tobytes()
+    """gpu_exhaustive_beam.py:85-88: the state's rows as sorted column
+    pairs, int16 bytes (the same point set gives the same signature
+    whichever of P and Q holds which point)."""
+    return _signatures(p[None, :], q[None, :])[0]
+
+
+def _signatures(p: np.ndarray, q: np.ndarray) -> list[bytes]:
+    """_signature of every row of P/Q arrays [count][n], in one sort."""
+    pairs = np.sort(np.stack((p, q), axis=2), axis=2).astype(np.int16)
+    return [row.tobytes() for row in pairs.reshape(len(pairs), -1)]
+
+
+class _BeamPick:
+    """The beam being chosen from materialised prospects: members in the
+    order they were taken, no two with one signature, and (when asked) none
+    whose signature an earlier generation already kept."""
+
+    def __init__(self, signature_of: dict[int, bytes], seen: set[bytes]):
+        self.signature_of = signature_of  # prospect index -> signature
+        self.seen = seen
+        self.members: list[int] = []
+        self.signatures: set[bytes] = set()
+
+    def fill(self, candidates, until: int, *, fresh_only: bool) -> None:
+        """Offers the candidates in order; stops right after the one that
+        brings the beam to `until` members.  Candidates that were never
+        materialised are passed over."""
+        for candidate in candidates:
+            sig = self.signature_of.get(candidate)
+            if sig is not None and sig not in self.signatures and not (fresh_only and sig in self.seen):
+                self.members.append(candidate)
+                self.signatures.add(sig)
+            if len(self.members) == until:
+                return
 
 
 def gpu_exhaustive_beam_search(initial: PairPermutations, *, seed: int, generations: int, elite_size: int,
@@ -137,61 +167,37 @@ def gpu_exhaustive_beam_search(initial: PairPermutations, *, seed: int, generati
         if not len(legal_order):
             break
 
-        quality_count = max(1, elite_size // 4)
-        quality_pool = list(map(int, legal_order[: min(len(legal_order), elite_size * 24)]))
-        diverse_pool = list(map(int, np.flatnonzero(valid & (scores <= best_score + energy_window))))
-        rng.shuffle(diverse_pool)
-        prospects = []
-        prospect_set: set[int] = set()
-        for index in quality_pool + diverse_pool[: elite_size * 24]:
-            if index not in prospect_set:
-                prospects.append(index)
-                prospect_set.add(index)
+        # prospects: the best 24 x beam legal children by score, then the
+        # first 24 x beam of the shuffled energy window (one rng call per
+        # generation, as gpu_exhaustive_beam.py:186-196 makes it)
+        reach = elite_size * 24
+        window = [int(i) for i in np.flatnonzero(valid & (scores <= best_score + energy_window))]
+        rng.shuffle(window)
+        prospects = list(dict.fromkeys([int(i) for i in legal_order[:reach]] + window[:reach]))
         prospect_desc = torch.from_numpy(np.ascontiguousarray(descriptors[prospects])).to(dev)
         prospect_p_dev, prospect_q_dev, _ = enumerate_switches(elite_p, elite_q, prospect_desc)
-        prospect_p, prospect_q = prospect_p_dev.cpu().numpy(), prospect_q_dev.cpu().numpy()
-        lookup = {candidate: position for position, candidate in enumerate(prospects)}
-        # rank the prospects by the elite key (gpu_exhaustive_beam.py:199-222)
+        # re-rank the prospects by the full elite key (score, max, sum b^2)
         prospect_bad = torch.empty((len(prospects), 2 * n), dtype=torch.int32, device=dev)
         score_population_device(prospect_p_dev, prospect_q_dev, badness=prospect_bad)
-        prospect_rank = rank_elites(scores[np.asarray(prospects, dtype=np.int64)], prospect_bad.cpu().numpy())
-        quality_pool = [prospects[int(position)] for position in prospect_rank]
-
-        selected: list[int] = []
-        selected_signatures: set[bytes] = set()
-
-        def retain(candidate: int, *, require_unseen: bool) -> bool:
-            position = lookup.get(candidate)
-            if position is None:
-                return False
-            signature = _signature(prospect_p[position], prospect_q[position])
-            if signature in selected_signatures or (require_unseen and signature in seen):
-                return False
-            selected.append(candidate)
-            selected_signatures.add(signature)
-            return True
-
-        for candidate in quality_pool:
-            retain(candidate, require_unseen=True)
-            if len(selected) == quality_count:
-                break
-        for candidate in diverse_pool:
-            retain(candidate, require_unseen=True)
-            if len(selected) == elite_size:
-                break
-        if len(selected) < elite_size:
-            for candidate in quality_pool + diverse_pool:
-                retain(candidate, require_unseen=False)
-                if len(selected) == elite_size:
-                    break
-        if not selected:
+        ranked = [prospects[int(i)] for i in
+                  rank_elites(scores[np.asarray(prospects, dtype=np.int64)], prospect_bad.cpu().numpy())]
+        signatures = _signatures(prospect_p_dev.cpu().numpy(), prospect_q_dev.cpu().numpy())
+        pick = _BeamPick(dict(zip(prospects, signatures)), seen)
+        pick.fill(ranked, max(1, elite_size // 4), fresh_only=True)
+        pick.fill(window, elite_size, fresh_only=True)
+        if len(pick.members) < elite_size:
+            pick.fill(ranked + window, elite_size, fresh_only=False)
+        if not pick.members:
+            # every prospect was seen before: forget the history, keep the best
             seen.clear()
-            selected = quality_pool[:elite_size]
-            selected_signatures = {_signature(prospect_p[lookup[c]], prospect_q[lookup[c]])
-                                   for c in selected if c in lookup}
-        seen.update(selected_signatures)
+            pick = _BeamPick(pick.signature_of, seen)
+            pick.members = ranked[:elite_size]
+            pick.signatures = {pick.signature_of[c] for c in pick.members}
+        selected = pick.members
+        seen.update(pick.signatures)
         if len(seen) > 500000:
-            seen = set(selected_signatures)
+            seen = set(pick.signatures)
+        lookup = {candidate: position for position, candidate in enumerate(prospects)}
 
         positions = torch.from_numpy(np.asarray([lookup[c] for c in selected], dtype=np.int64)).to(dev)
         elite_p, elite_q, elite_bad, elite_scores, elite_bad_host = _score_rank(

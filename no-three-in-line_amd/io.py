@@ -1,138 +1,180 @@
-"""Candidate files in the reference's formats (io.py:1-132).
+"""Candidate files: read and write the three on-disk forms of a point set.
 
-- JSON (strict): an object {"n": n, "points": [[row, column], ...]} or
-  {"n": n, "P": [...], "Q": [...]} with free-form metadata keys, or a bare
-  list of points.  Coordinates must be genuine integers (no bool, float or
-  string aliases) and n a positive integer; nothing is coerced or shifted.
-- Compact code: one token, a symmetry prefix character then two column
-  characters per row from the reference's 90-character alphabet.
-- Plain text (permissive): every integer in the text, taken in pairs.
+Format specification (what files written by the reference's tools look like,
+reference io.py:1-132; this module is written from this specification):
+
+1. Grid size.  A grid size is a Python int >= 1; bool, float and str values
+   are refused ("... must be a positive integer").  The size is taken from,
+   in this order of precedence: the caller's explicit ``n``; the size the
+   file declares (JSON key "n"); one more than the largest row among the
+   points.  An explicit ``n`` and a declared "n" are both checked even when
+   the explicit one wins.
+2. Which form.  A file is JSON when its suffix is ``.json`` (any case) or
+   its stripped text starts with "{" or "[".  Otherwise it is a compact
+   code when it is a single whitespace-free token starting with a symmetry
+   prefix; anything else is plain text.
+3. JSON.  Either a bare list of points, or an object with "points", or an
+   object with list-valued "P" and "Q" (row r holds the points (r, P[r])
+   and (r, Q[r]); the P points come first).  A point is a list of exactly
+   two genuine ints; nothing is coerced, rounded or shifted.  Every key of
+   an object except "points" is returned as metadata.
+4. Compact code.  One prefix character naming the symmetry class, then two
+   characters per row (row 0 first, each row's columns in increasing
+   order) from a fixed 90-character alphabet, so that slot i of the payload
+   is a point of row i // 2.  The size is half the payload length unless
+   given.
+5. Plain text.  Every (possibly negative) decimal integer in the text, in
+   order, taken two at a time as (row, column); an odd count is an error.
+6. Writing.  JSON with keys "n", "points" (sorted), then the caller's
+   metadata, indented by 2, with a trailing newline.
+
+The alphabet, the prefixes and their names are part of the file format and
+must be the reference's characters (io.py:9-21) for codes to be
+interchangeable.
 """
 from __future__ import annotations
 
 import json
 import re
+from collections import Counter
 from pathlib import Path
 from typing import Iterable
 
 from .state import Point
 
-# The compact alphabet and symmetry prefixes are a file format: they must be
-# the reference's characters (io.py:9-21) for codes to be interchangeable.
 ALPHABET = "0123456789ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz#$%&@?!()[]<>{}=*+|-/~^_:;,."
 SYMMETRY_PREFIXES = ".:/-ocx+*"
-SYMMETRY_NAMES = {".": "iden", ":": "rot2", "/": "dia1", "-": "ort1", "o": "rot4", "c": "rct4", "x": "dia2",
-                  "+": "ort2", "*": "full"}
-_COLUMN_OF = {ch: value for value, ch in enumerate(ALPHABET)}
+SYMMETRY_NAMES = dict(zip(SYMMETRY_PREFIXES, ("iden", "rot2", "dia1", "ort1", "rot4", "rct4", "dia2", "ort2", "full")))
+_PREFIX_OF = {name: prefix for prefix, name in SYMMETRY_NAMES.items()}
+_VALUE_OF = {ch: i for i, ch in enumerate(ALPHABET)}
+_INTEGER = re.compile(r"-?\d+")
 
 
-def _grid_n(value: object, label: str = "n") -> int:
-    """A positive int, and only an int (io.py:25-28)."""
+def _positive_int(value: object, what: str = "n") -> int:
+    """Rule 1: exactly an int (no bool or other subclass) of at least 1."""
     if type(value) is not int or value < 1:
-        raise ValueError(f"{label} must be a positive integer")
+        raise ValueError(f"{what} must be a positive integer")
     return value
 
 
-def _pairs(value: object) -> list[Point]:
-    """io.py:31-40: a list of exactly-two-integer lists."""
-    if not isinstance(value, list):
+def _strict_points(items: object) -> list[Point]:
+    """Rule 3: a list whose items are lists of exactly two genuine ints."""
+    if not isinstance(items, list):
         raise ValueError("JSON points must be a list")
-    points: list[Point] = []
-    for index, pair in enumerate(value):
-        if not isinstance(pair, list) or len(pair) != 2 or any(type(v) is not int for v in pair):
-            raise ValueError(f"point {index} must be a list of exactly two integers")
-        points.append((pair[0], pair[1]))
-    return points
+
+    bad = next((i for i, item in enumerate(items)
+                if not (isinstance(item, list) and len(item) == 2 and all(type(x) is int for x in item))), None)
+    if bad is not None:
+        raise ValueError(f"point {bad} must be a list of exactly two integers")
+    return [(item[0], item[1]) for item in items]
+
+
+def _size(explicit: int | None, declared: int | None, points: list[Point]) -> int:
+    """Rule 1's precedence; the result is checked again."""
+    if explicit is not None:
+        return _positive_int(explicit)
+    if declared is not None:
+        return _positive_int(declared)
+    return _positive_int(1 + max((row for row, _ in points), default=-1))
+
+
+def _is_json(path: Path, text: str) -> bool:
+    return path.suffix.lower() == ".json" or text[:1] in ("{", "[")
+
+
+def _read_json(text: str) -> tuple[int | None, list[Point], dict]:
+    """(declared n, points, metadata) of a JSON candidate (rule 3)."""
+    document = json.loads(text)
+    if isinstance(document, list):
+        return None, _strict_points(document), {}
+    if not isinstance(document, dict):
+        raise ValueError("JSON candidate must be an object or a points list")
+    declared = _positive_int(document["n"], "JSON n") if "n" in document else None
+    items = document.get("points")
+    if items is None:
+        if "P" not in document or "Q" not in document:
+            raise ValueError("JSON candidate has neither points nor P/Q")
+        layers = document["P"], document["Q"]
+        if not all(isinstance(layer, list) for layer in layers):
+            raise ValueError("JSON P and Q must be lists")
+        items = [[row, column] for layer in layers for row, column in enumerate(layer)]
+    extra = {key: value for key, value in document.items() if key != "points"}
+    return declared, _strict_points(items), extra
+
+
+def _read_text(text: str) -> list[Point]:
+    """Rule 5."""
+    values = [int(token) for token in _INTEGER.findall(text)]
+    if len(values) % 2:
+        raise ValueError("text candidate contains an odd number of integers")
+    it = iter(values)
+    return list(zip(it, it))
 
 
 def decode_compact(code: str, n: int | None = None) -> tuple[int, str, list[Point]]:
-    """io.py:44-67: (n, symmetry name, points) of a compact code; slot i of
-    the payload is a point of row i // 2."""
+    """Rule 4: (n, symmetry name, points) of a compact code."""
     if n is not None:
-        n = _grid_n(n)
+        _positive_int(n)
     code = code.strip()
-    if not code or code[0] not in SYMMETRY_PREFIXES:
+    prefix, payload = code[:1], code[1:]
+    if prefix == "" or prefix not in SYMMETRY_NAMES:
         raise ValueError("compact code has no recognized symmetry prefix")
-    payload = code[1:]
     if n is None:
-        if len(payload) % 2:
+        if len(payload) & 1:
             raise ValueError("compact payload length is not even")
-        n = len(payload) // 2
-    n = _grid_n(n)
+        n = _positive_int(len(payload) // 2)
     if len(payload) != 2 * n:
         raise ValueError(f"compact payload has {len(payload)} columns, expected {2 * n}")
     points: list[Point] = []
-    for offset, ch in enumerate(payload):
-        column = _COLUMN_OF.get(ch)
-        if column is None:
+    for slot, ch in enumerate(payload):
+        if ch not in _VALUE_OF:
             raise ValueError(f"unknown compact character {ch!r}")
-        if column >= n:
-            raise ValueError(f"column {column} outside n={n}")
-        points.append((offset // 2, column))
-    return n, SYMMETRY_NAMES[code[0]], points
+        if _VALUE_OF[ch] >= n:
+            raise ValueError(f"column {_VALUE_OF[ch]} outside n={n}")
+        points.append((slot >> 1, _VALUE_OF[ch]))
+    return n, SYMMETRY_NAMES[prefix], points
 
 
 def encode_compact(points: Iterable[Point], n: int, symmetry: str = "iden") -> str:
-    """io.py:70-82: the compact code of a two-points-per-row candidate (each
-    row's columns in increasing order)."""
-    prefix = {name: p for p, name in SYMMETRY_NAMES.items()}.get(symmetry)
-    if prefix is None and symmetry in SYMMETRY_PREFIXES:
-        prefix = symmetry
+    """Rule 4 in the other direction; `symmetry` is a name or a prefix."""
+    prefix = _PREFIX_OF.get(symmetry) or (symmetry if symmetry in SYMMETRY_NAMES else None)
     if prefix is None:
         raise ValueError(f"unknown symmetry {symmetry!r}")
-    rows: list[list[int]] = [[] for _ in range(n)]
-    for row, column in points:
+    ordered = sorted(points)
+    for row, column in ordered:
         if not (0 <= row < n and 0 <= column < n):
             raise ValueError(f"point {(row, column)} outside n={n}")
-        rows[row].append(column)
-    if any(len(r) != 2 for r in rows):
+    per_row = Counter(row for row, _ in ordered)
+    if len(ordered) != 2 * n or any(per_row[row] != 2 for row in range(n)):
         raise ValueError("compact format requires exactly two points in every row")
-    return prefix + "".join(ALPHABET[c] for r in rows for c in sorted(r))
+    # sorted (row, column) pairs are already row by row, columns increasing
+    return prefix + "".join(ALPHABET[column] for _, column in ordered)
 
 
 def load_candidate(path, n: int | None = None) -> tuple[int, list[Point], dict]:
-    """io.py:85-126: (n, points, metadata) from a JSON, compact or text
-    candidate.  An explicit n overrides the file's (after both are checked);
-    otherwise the file's n, else one more than the largest row."""
+    """(n, points, metadata) of a candidate file in any of the three forms;
+    metadata always holds "source_path"."""
     if n is not None:
-        n = _grid_n(n)
-    candidate = Path(path)
-    text = candidate.read_text(encoding="utf-8").strip()
-    metadata: dict = {"source_path": str(candidate)}
-    if candidate.suffix.lower() == ".json" or text.startswith("{") or text.startswith("["):
-        payload = json.loads(text)
-        if isinstance(payload, dict):
-            file_n = _grid_n(payload["n"], "JSON n") if "n" in payload else None
-            raw = payload.get("points")
-            if raw is None and "P" in payload and "Q" in payload:
-                if not isinstance(payload["P"], list) or not isinstance(payload["Q"], list):
-                    raise ValueError("JSON P and Q must be lists")
-                raw = [[r, c] for r, c in enumerate(payload["P"])] + [[r, c] for r, c in enumerate(payload["Q"])]
-            if raw is None:
-                raise ValueError("JSON candidate has neither points nor P/Q")
-            metadata.update({k: v for k, v in payload.items() if k != "points"})
-        elif isinstance(payload, list):
-            raw, file_n = payload, None
-        else:
-            raise ValueError("JSON candidate must be an object or a points list")
-        points = _pairs(raw)
-        inferred = max((row for row, _ in points), default=-1) + 1
-        return _grid_n(n if n is not None else file_n if file_n is not None else inferred), points, metadata
+        _positive_int(n)
+    source = Path(path)
+    text = source.read_text(encoding="utf-8").strip()
+    metadata: dict = {"source_path": str(source)}
+    if _is_json(source, text):
+        declared, points, extra = _read_json(text)
+        metadata.update(extra)
+        return _size(n, declared, points), points, metadata
     tokens = text.split()
-    if len(tokens) == 1 and tokens[0][0] in SYMMETRY_PREFIXES:
+    if len(tokens) == 1 and tokens[0][0] in SYMMETRY_NAMES:
         size, symmetry, points = decode_compact(tokens[0], n=n)
-        metadata.update({"compact": tokens[0], "symmetry": symmetry})
+        metadata["compact"] = tokens[0]
+        metadata["symmetry"] = symmetry
         return size, points, metadata
-    values = [int(v) for v in re.findall(r"-?\d+", text)]
-    if len(values) % 2:
-        raise ValueError("text candidate contains an odd number of integers")
-    points = list(zip(values[::2], values[1::2]))
-    inferred = max((row for row, _ in points), default=-1) + 1
-    return _grid_n(n if n is not None else inferred), points, metadata
+    points = _read_text(text)
+    return _size(n, None, points), points, metadata
 
 
 def write_candidate(path, n: int, points: Iterable[Point], **metadata: object) -> None:
-    """io.py:129-132: sorted points, then the metadata keys."""
-    ordered = sorted((int(r), int(c)) for r, c in points)
-    payload = {"n": int(n), "points": [[r, c] for r, c in ordered], **metadata}
-    Path(path).write_text(json.dumps(payload, indent=2, sort_keys=False) + "\n", encoding="utf-8")
+    """Rule 6."""
+    document = {"n": int(n), "points": [[r, c] for r, c in sorted((int(r), int(c)) for r, c in points)]}
+    document.update(metadata)
+    Path(path).write_text(json.dumps(document, indent=2) + "\n", encoding="utf-8")

@@ -7,8 +7,10 @@ step is one scoring pass over the whole population.
 
 Also reported: the same population's kernel roofline (INT32 VALU), a CPU
 baseline (the oracle's OpenMP restatement of the reference loop, timed on the
-host's cores on a bounded sample), and the replica-exchange tempering rate
-(64 ladders x 64 temperatures at n=75, proposals per second).
+host's cores on a bounded sample), the replica-exchange tempering rate (64
+ladders x 64 temperatures at n=75) and the batched uneven-size rate, each in
+proposals per second with its walk kernel's roofline and a CPU replay
+baseline, the learned-proposal A/B, the search engines and the move scores.
 """
 from __future__ import annotations
 
@@ -54,6 +56,24 @@ def determinant_ops_per_state(n: int) -> int:
     """6 C(m,3) + 2 C(m,2): the reference loop's integer work (accelerated.py:27-36)."""
     m = 2 * n
     return 6 * (m * (m - 1) * (m - 2) // 6) + 2 * (m * (m - 1) // 2)
+
+
+def walk_ops_per_valid_proposal(n: int) -> int:
+    """Algorithmic work of the walk's exact delta for one valid proposal
+    (the rejection path, which 97-98% of valid proposals take): for each of
+    the m - 2 unmoved slots, the histograms seen from u' and from v' in the
+    moved state cost a key difference, a table lookup, an increment that
+    returns the rank, a clear and an accumulate each (5 + 5), and joint(u, v)
+    in the old state a key difference, a lookup and a compare (3):
+    13 (m - 2) integer operations (1,924 at n=75)."""
+    return 13 * (2 * n - 2)
+
+
+def _walk_roofline(valid: float, ops_weighted: float, kernel_ms: float) -> dict:
+    achieved = ops_weighted / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "TOP/s",
+            "frac": achieved / PEAK_VALU_TOPS, "traffic": None, "kernel_ms": kernel_ms,
+            "valid_proposals_per_launch": valid}
 
 
 def _profile_values() -> dict:
@@ -126,11 +146,28 @@ def tempering_rate(pkg, torch, sweeps: int = 10000) -> dict:
     after = chains.proposals.cpu().numpy()
     chains.validate(range(0, replicas, 97))
     proposals = sweeps * steps * replicas
-    return {"replicas": replicas, "ladders": replicas // ladder, "sweeps": sweeps, "steps_per_sweep": steps,
-            "seconds": elapsed, "proposals_per_s": proposals / elapsed,
-            "valid_proposals_per_s": float(after[0] - before[0]) / elapsed,
-            "accepted_per_s": float(after[1] - before[1]) / elapsed,
-            "best_E": int(chains.best_scores.min().item())}
+    out = {"replicas": replicas, "ladders": replicas // ladder, "sweeps": sweeps, "steps_per_sweep": steps,
+           "seconds": elapsed, "proposals_per_s": proposals / elapsed,
+           "valid_proposals_per_s": float(after[0] - before[0]) / elapsed,
+           "accepted_per_s": float(after[1] - before[1]) / elapsed,
+           "best_E": int(chains.best_scores.min().item())}
+    # roofline of the walk kernel: HIP events on the launch stream around
+    # sweeps without exchanges (one walk launch and one single-wave counter
+    # bump per sweep), after the timed run
+    k = 192  # six replays of the 32-sweep graph
+    chains.run(32, steps, 0)  # captures the graph outside the timed window
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    before = chains.proposals.cpu().numpy().copy()
+    ev0.record(stream)
+    chains.run(k, steps, 0)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    valid = float(chains.proposals.cpu().numpy()[0] - before[0]) / k
+    out["roofline"] = _walk_roofline(valid, valid * walk_ops_per_valid_proposal(N), ev0.elapsed_time(ev1) / k)
+    out["roofline"]["ops_per_valid_proposal"] = walk_ops_per_valid_proposal(N)
+    return out
 
 
 def batched_rate(pkg, torch, sweeps: int = 1000) -> dict:
@@ -156,10 +193,59 @@ def batched_rate(pkg, torch, sweeps: int = 1000) -> dict:
     proposals = sum(g.chains.temperatures_host.size * g.steps for g in groups) * sweeps
     for g in groups:
         g.chains.validate(range(0, len(g.chains.temperatures_host), 211))
-    return {"sizes": {str(n): r for n, r in sizes.items()}, "sweeps": sweeps, "seconds": elapsed,
-            "proposals_per_s": proposals / elapsed, "block_waves": {str(n): w for n, w in shapes.items()},
-            "autotune_seconds": tune_seconds,
-            "best_E": {str(g.chains.n): int(g.chains.best_scores.min().item()) for g in groups}}
+    out = {"sizes": {str(n): r for n, r in sizes.items()}, "sweeps": sweeps, "seconds": elapsed,
+           "proposals_per_s": proposals / elapsed, "block_waves": {str(n): w for n, w in shapes.items()},
+           "autotune_seconds": tune_seconds,
+           "best_E": {str(g.chains.n): int(g.chains.best_scores.min().item()) for g in groups}}
+    # roofline of walk_batched_kernel: HIP events around sweeps without
+    # exchanges (one batched walk launch and one single-wave counter bump per
+    # sweep), after the timed run
+    k = 32  # two replays of the 16-sweep graph
+    plain = [pkg.BatchGroup(g.chains, 0, g.steps) for g in groups]
+    pkg.run_batched(plain, 16)  # captures the graph outside the timed window
+    torch.cuda.synchronize()
+    before = [g.chains.proposals.cpu().numpy()[0] for g in groups]
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    pkg.run_batched(plain, k)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    valid = [float(g.chains.proposals.cpu().numpy()[0] - b) / k for g, b in zip(groups, before)]
+    ops = sum(v * walk_ops_per_valid_proposal(g.chains.n) for v, g in zip(valid, groups))
+    out["roofline"] = _walk_roofline(sum(valid), ops, ev0.elapsed_time(ev1) / k)
+    out["roofline"]["ops_per_valid_proposal"] = {str(g.chains.n): walk_ops_per_valid_proposal(g.chains.n)
+                                                 for g in groups}
+    return out
+
+
+def batched_cpu_baseline(pkg, oracle, replicas_per_size: int = 256) -> dict:
+    """The oracle's replay of the reference walk for every size of the
+    batched workload: one sweep of 2n proposals for `replicas_per_size`
+    seeded replicas of each n (ladders of 64 temperatures 0.05..2.0),
+    OpenMP over replicas on all host cores; the config's rate is its
+    proposals over the time the measured per-size rates give it."""
+    import numpy as np
+
+    sizes = {40: 8192, 52: 4096, 61: 4096, 75: 2048, 83: 2048, 100: 1024}
+    cores = oracle.cpu_count()
+    per_size, seconds_per_sweep, proposals_per_sweep = {}, 0.0, 0
+    for n, full in sizes.items():
+        p, q = oracle.generate_states(40 + n, n, replicas_per_size)
+        cols = np.stack((p, q), axis=2).reshape(replicas_per_size, 2 * n)
+        temps = np.tile(np.geomspace(0.05, 2.0, 64), replicas_per_size // 64).astype(np.float32)
+        replay = oracle.TemperingReplay(cols, temps, 7, k_move=pkg.tempering.move_table_size(temps), k_exchange=2)
+        t0 = time.perf_counter()
+        replay.advance(2 * n, threads=cores)
+        rate = replicas_per_size * 2 * n / (time.perf_counter() - t0)
+        per_size[str(n)] = rate
+        seconds_per_sweep += full * 2 * n / rate
+        proposals_per_sweep += full * 2 * n
+    return {"value": proposals_per_sweep / seconds_per_sweep, "unit": "proposals/s", "cores": cores, "kind": "port",
+            "sample": f"one sweep of 2n proposals for {replicas_per_size} replicas of each n, oracle C replay of "
+                      f"tempering.walk with its pair-loop delta, OpenMP over replicas; the rate of the full "
+                      f"mix weighted by its replica counts",
+            "per_size_proposals_per_s": per_size}
 
 
 def _ab_summary(res: dict) -> dict:
@@ -365,7 +451,10 @@ def tempering_cpu_baseline(pkg, oracle, seconds: float = 10.0) -> dict:
     cols = np.stack((p, q), axis=2).reshape(replicas, 2 * N)
     temps = np.tile(np.geomspace(0.05, 2.0, ladder), replicas // ladder).astype(np.float32)
     k_move = pkg.tempering.move_table_size(temps)
-    replay = oracle.TemperingReplay(cols, temps, 3, k_move=k_move, k_exchange=2)
+    import torch
+
+    chains = pkg.TemperingChains(cols, temps, seed=3)
+    replay = oracle.TemperingReplay(cols, temps, 3, k_move=k_move, k_exchange=chains._exchange_table(ladder)[2])
     t0 = time.perf_counter()
     replay.advance(steps, threads=cores)
     sweeps = 1
@@ -379,21 +468,26 @@ def tempering_cpu_baseline(pkg, oracle, seconds: float = 10.0) -> dict:
     t1 = time.perf_counter()
     single.advance(steps, threads=1)
     single_rate = 256 * steps / (time.perf_counter() - t1)
-    # the device walk over the same sweeps, states, temperatures and seed:
-    # every replica's trajectory must match the replay's
-    import torch
-
-    chains = pkg.TemperingChains(cols, temps, seed=3)
+    # the device walk over the same sweeps, states, temperatures and seed,
+    # then both continue with alternating sweeps and exchanges: every
+    # replica's trajectory must match the replay's
     for _ in range(sweeps):
         chains.advance(steps)
+    names = ("columns", "badness", "scores", "best_columns", "best_scores", "accepted", "removed_points")
+    exact = all(np.array_equal(getattr(chains, name).cpu().numpy(), getattr(replay, name)) for name in names)
+    for _ in range(2):
+        exact &= chains.exchange(ladder) == replay.exchange(ladder)
+        chains.advance(steps)
+        replay.advance(steps, threads=cores)
+    exact &= chains.exchange(ladder) == replay.exchange(ladder)
     torch.cuda.synchronize()
-    exact = all(np.array_equal(getattr(chains, name).cpu().numpy(), getattr(replay, name))
-                for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted",
-                             "removed_points"))
+    exact &= all(np.array_equal(getattr(chains, name).cpu().numpy(), getattr(replay, name)) for name in names)
     return {"value": sweeps * steps * replicas / elapsed, "unit": "proposals/s", "cores": cores, "kind": "port",
             "sample": f"{sweeps} sweep(s) x {steps} proposals x {replicas} replicas (config 3's ladder), oracle C "
                       f"replay of tempering.walk with its pair-loop delta, OpenMP over replicas",
-            "single_thread_proposals_per_s": single_rate, "trajectory_exact": exact}
+            "single_thread_proposals_per_s": single_rate, "trajectory_exact": exact,
+            "trajectory_checked": f"{sweeps} sweeps, then 3 exchanges and 2 more sweeps alternating, all "
+                                  f"{replicas} replicas"}
 
 
 def move_cpu_baseline(pkg, oracle) -> dict:
@@ -565,6 +659,11 @@ def main() -> int:
             if not t_cpu["trajectory_exact"]:
                 log("bench: PARITY FAILURE in the tempering trajectories")
                 exact = False
+        if "batched_uneven" in extra:
+            log("bench: batched CPU baseline")
+            b_cpu = batched_cpu_baseline(pkg, oracle)
+            extra["batched_uneven"]["cpu_baseline"] = b_cpu
+            extra["batched_uneven"]["gpu_over_cpu"] = extra["batched_uneven"]["proposals_per_s"] / b_cpu["value"]
         if "move_scores" in extra:
             moves_cpu = move_cpu_baseline(pkg, oracle)
             extra["move_scores"]["cpu_baseline"] = moves_cpu

@@ -135,7 +135,8 @@ typedef struct n3_chains {
     int64_t* proposals;      /* [2]: valid proposals (total), accepted moves (total) */
     int block_waves;         /* replicas (wavefronts) per workgroup; 0 = automatic */
     int32_t* log;            /* optional proposal log (NULL = off): per valid proposal
-                                (badness u, badness v, energy, slot, delta, accepted);
+                                8 int32 (badness u, badness v, energy, slot, delta,
+                                accepted, u | v << 16, column u | column v << 16);
                                 each replica reserves `steps` consecutive entries per
                                 sweep (entry = reservation + step); invalid proposals
                                 and reservations past the capacity write nothing */
@@ -259,6 +260,47 @@ int n3_metropolis_thresholds64(const float* host_temperatures, int replicas, int
 int n3_temper_run_learned(const n3_chains* chains, const uint32_t* weights, const int32_t* temp_index,
                           int max_energy, const uint64_t* thr64, int k64, int sweeps, int steps, int ladder,
                           uint64_t seed, void* stream);
+
+/* ---- point-network proposals (no reference counterpart) ----------------
+ * The network scores a point from 8 features: its collinear-pair count
+ * (b/32 up to 63, min(b, 7)/4, b > 0), its row/128 and column/128, ln T/4
+ * (temperature code lt_q = round(32 ln T) per replica) and the state's
+ * energy (E/256 up to 510, E/64 up to 63), each a multiple of 1/128.  With
+ * W1 and b1 on the same grid (n3_mlp_adam keeps them there), the hidden
+ * layer's fp32 MFMA sums are exact and the output layer is a fixed-order fma
+ * chain, so a CPU replays every weight bit for bit; a point's weight is
+ * expw[clamp(floor(64 g), -384, 352) + 384].  n <= 127. */
+
+/* expw (uint32[737]): clamp(round(128 exp(q / 64)), 1, 2^15), q = -384..352.
+ * Host. */
+int n3_point_exp_table(uint32_t* host_out);
+
+/* Walk constants (float[260]) from the bf16 parameters: b1 and w2 in the
+ * order the MFMA output lanes read them, then b2.  N3_EINVAL when W1 or b1
+ * is off the 1/128 grid.  Host. */
+int n3_point_constants(const uint16_t* host_params, float* host_consts);
+
+/* Integer weight of every slot of `count` states (columns, badness int32
+ * [count][2n], scores [count], lt_q [count]) into weights int32[count][2n],
+ * from the same device function the walk uses. */
+int n3_point_weights(const int32_t* columns, const int32_t* badness, const int32_t* scores, const int32_t* lt_q,
+                     int n, int64_t count, const uint16_t* params, const float* consts, const uint32_t* expw,
+                     int32_t* weights, void* stream);
+
+/* `sweeps` x (point-proposal walk of `steps` proposals; exchange(ladder) if
+ * ladder > 1), graph-captured.  A proposal is, with probability 0.2, a
+ * uniform pair of slots, otherwise both slots drawn in proportion to the
+ * point weights of the current state (64-bit inverse-CDF targets).
+ * Acceptance is delayed: the Metropolis test on the exact integer delta
+ * (Philox word < thr64(delta)), then, for moves that pass it, the moved
+ * state's weights and the proposal-ratio test h N S1'^2 < 2^32 N' S1^2
+ * (N = S1^2 + 4 m^2 w(u) w(v)), i.e. acceptance min(1, e^(-delta/T))
+ * min(1, q'/q).  Philox counters (step, sweep, slot, "LRND") and
+ * (step, sweep, slot, "HAST").  No counterpart in the reference (the
+ * nearest is bad_bias, tempering.py:51-55). */
+int n3_temper_run_learned_point(const n3_chains* chains, const uint16_t* params, const float* consts,
+                                const uint32_t* expw, const int32_t* lt_q, const uint64_t* thr64, int k64, int sweeps,
+                                int steps, int ladder, uint64_t seed, void* stream);
 
 /* Drops the learned walk's cached graphs. */
 void n3_learned_release(void);

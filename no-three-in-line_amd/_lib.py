@@ -117,6 +117,11 @@ SIGNATURES = {
     "n3_metropolis_thresholds64": (_i, [_vp, _i, _i, _vp]),
     "n3_temper_run_learned": (_i, [ctypes.POINTER(Chains), _vp, _vp, _i, _vp, _i, _i, _i, _i, _u64, _vp]),
     "n3_learned_release": (None, []),
+    "n3_point_exp_table": (_i, [_vp]),
+    "n3_point_constants": (_i, [_vp, _vp]),
+    "n3_point_weights": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "n3_temper_run_learned_point": (_i, [ctypes.POINTER(Chains), _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _u64,
+                                         _vp]),
     "n3_exchange_thresholds": (_i, [_vp, _i, _i, _i, _vp, _vp]),
 }
 

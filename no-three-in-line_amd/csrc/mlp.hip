@@ -11,41 +11,11 @@
 // The weight gradient dW1 = dH X is a second MFMA product with the rows as K.
 #include <mutex>
 
+#include "mlp_common.h"
 #include "runtime.h"
 #include "wave_ops.h"
 
 namespace no3 {
-
-constexpr int kIn = 8;
-constexpr int kHidden = 128;
-constexpr int kW1 = 0;
-constexpr int kB1 = kHidden * kIn;
-constexpr int kW2 = kB1 + kHidden;
-constexpr int kB2 = kW2 + kHidden;
-constexpr int kParams = kB2 + 1;  // 1281
-
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
-typedef float floatx16_t __attribute__((ext_vector_type(16)));
-
-__host__ __device__ inline float bf16_to_float(uint16_t h) {
-  union {
-    uint32_t u;
-    float f;
-  } v;
-  v.u = (uint32_t)h << 16;
-  return v.f;
-}
-
-// round to nearest even (finite inputs)
-__host__ __device__ inline uint16_t float_to_bf16(float f) {
-  union {
-    uint32_t u;
-    float f;
-  } v;
-  v.f = f;
-  const uint32_t lsb = (v.u >> 16) & 1u;
-  return (uint16_t)((v.u + 0x7FFFu + lsb) >> 16);
-}
 
 // Features of a candidate point with collinear-pair count b at temperature
 // exp(log_t) in a state of energy e: b in four encodings, the temperature,
@@ -62,10 +32,6 @@ __host__ __device__ inline void point_features(int b, float log_t, int e, float 
   f[6] = sqrtf(fe) * (1.0f / 12.0f);
   f[7] = fb * 0.125f * log_t;
 }
-
-// Hidden unit held by accumulator element r of tile t in lane half `half`
-// for the 32x32 MFMA output layout (row = 8 (r/4) + 4 half + r%4).
-__device__ inline int hidden_of(int t, int r, int half) { return 32 * t + 8 * (r >> 2) + 4 * half + (r & 3); }
 
 // Pre-activations of the 32 rows whose features this lane supplies:
 // lane l provides row (l % 32), features 4 (l / 32) .. +3.
@@ -387,7 +353,9 @@ __global__ void adam_kernel(float* __restrict__ master, uint16_t* __restrict__ p
   const float vhat = vi / (1.0f - powf(beta2, (float)step));
   const float w = master[i] - lr * mhat / (sqrtf(vhat) + eps);
   master[i] = w;
-  params[i] = float_to_bf16(w);
+  // W1 and b1 go onto the fixed-point grid of the point network's exact
+  // inference (multiples of 1/128, |k| <= 255); w2 and b2 are plain bf16
+  params[i] = i < kW2 ? grid_bf16(w) : float_to_bf16(w);
 }
 
 // Proposal weights for every (temperature t, energy e, count b):

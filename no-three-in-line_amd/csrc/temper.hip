@@ -146,7 +146,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     if (lane == 0) first = atomicAdd(a.log_count, (unsigned long long)a.steps);
     first = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(first >> 32), 0) << 32) |
             (uint32_t)__builtin_amdgcn_readlane((int)first, 0);
-    if (first + (unsigned long long)a.steps <= (unsigned long long)a.log_capacity) log_base = a.log + 6 * first;
+    if (first + (unsigned long long)a.steps <= (unsigned long long)a.log_capacity) log_base = a.log + kLogFields * first;
   }
 
   // Philox outputs of 64 consecutive steps, one step per lane, computed on
@@ -294,15 +294,17 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       if (!accept) accept = r[2] < register_table(thr_lo, thr_hi, thr, delta < a.k_move ? delta : a.k_move - 1);
     }
     if (log_base && lane == 0) {
-      // (badness u, badness v, energy, slot, delta, accepted): training data
-      // for the proposal network
-      int32_t* e = log_base + 6 * step;
+      // (badness u, badness v, energy, slot, delta, accepted, u | v << 16,
+      // column u | column v << 16): training data for the proposal network
+      int32_t* e = log_base + kLogFields * step;
       e[0] = bad_u;
       e[1] = bad_v;
       e[2] = score;
       e[3] = slot;
       e[4] = delta;
       e[5] = accept ? 1 : 0;
+      e[6] = u | (v << 16);
+      e[7] = cu | (cv << 16);
     }
     if (!accept) continue;
 

@@ -5,6 +5,9 @@
 
 namespace no3 {
 
+// int32 fields of one proposal-log entry
+constexpr int kLogFields = 8;
+
 struct WalkArgs {
   int n;
   int replicas;
@@ -23,7 +26,7 @@ struct WalkArgs {
   int k_move;
   const uint32_t* counters;
   unsigned long long* proposals;
-  int32_t* log;                   // optional proposal log, 6 int32 per entry
+  int32_t* log;                   // optional proposal log, kLogFields int32 per entry
   long long log_capacity;
   unsigned long long* log_count;
   const float* centers;           // umbrella potentials: null strengths = none

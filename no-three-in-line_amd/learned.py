@@ -41,6 +41,58 @@ def from_bf16(h: np.ndarray) -> np.ndarray:
     return (np.asarray(h, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
+GRID_SCALE, GRID_MAX = 128, 255
+W2_OFFSET = 8 * 128 + 128  # W1 and b1 come first in the parameter vector
+
+
+def network_bf16(master: np.ndarray) -> np.ndarray:
+    """bf16 parameters from fp32 master weights: W1 and b1 rounded onto the
+    grid of multiples of 1/128 in (-2, 2) (what the Adam kernel writes, and
+    what the point network's exact inference needs), w2 and b2 to bf16."""
+    master = np.ascontiguousarray(master, dtype=np.float32)
+    out = to_bf16(master)
+    k = np.clip(np.rint(master[:W2_OFFSET] * np.float32(GRID_SCALE)), -GRID_MAX, GRID_MAX).astype(np.float32)
+    out[:W2_OFFSET] = to_bf16(k * np.float32(1.0 / GRID_SCALE))
+    return out
+
+
+def temperature_codes(temperatures) -> np.ndarray:
+    """lt_q = round(32 ln T) per replica (the point network's temperature
+    feature is lt_q / 128)."""
+    t = np.asarray(temperatures, dtype=np.float32).astype(np.float64)
+    return np.clip(np.rint(32.0 * np.log(t)), -GRID_MAX, GRID_MAX).astype(np.int32)
+
+
+def point_codes(counts, rows, cols, lt_q, energy) -> np.ndarray:
+    """Feature codes [N, 8] of candidate points for the point network
+    (csrc/mlp_common.h point_codes); the features are codes / 128."""
+    b = np.asarray(counts, dtype=np.int64)
+    shape = b.shape
+    e = np.broadcast_to(np.asarray(energy, dtype=np.int64), shape)
+    out = np.empty(shape + (8,), dtype=np.int64)
+    out[..., 0] = 4 * np.minimum(b, 63)
+    out[..., 1] = 32 * np.minimum(b, 7)
+    out[..., 2] = np.where(b > 0, GRID_SCALE, 0)
+    out[..., 3] = np.broadcast_to(np.asarray(rows, dtype=np.int64), shape)
+    out[..., 4] = np.broadcast_to(np.asarray(cols, dtype=np.int64), shape)
+    out[..., 5] = np.broadcast_to(np.asarray(lt_q, dtype=np.int64), shape)
+    out[..., 6] = np.minimum(e, 510) >> 1
+    out[..., 7] = 2 * np.minimum(e, 63)
+    return out
+
+
+def point_codes_device(counts: torch.Tensor, rows: torch.Tensor, cols: torch.Tensor, lt_q: torch.Tensor,
+                       energy: torch.Tensor) -> torch.Tensor:
+    """point_codes on device tensors (int [N] each) -> bf16 bits [N, 8]
+    (int16), exact."""
+    b = counts.to(torch.int32)
+    e = energy.to(torch.int32)
+    codes = torch.stack((4 * torch.clamp(b, max=63), 32 * torch.clamp(b, max=7), (b > 0).to(torch.int32) * GRID_SCALE,
+                         rows.to(torch.int32), cols.to(torch.int32), lt_q.to(torch.int32),
+                         torch.clamp(e, max=510) // 2, 2 * torch.clamp(e, max=63)), dim=1)
+    return (codes.to(torch.float32) / GRID_SCALE).to(torch.bfloat16).view(torch.int16).contiguous()
+
+
 def point_features(counts, log_t, energy) -> np.ndarray:
     """float32 features [N, 8] of candidate points (the formula of
     point_features in csrc/mlp.hip)."""
@@ -85,7 +137,7 @@ class ProposalNetwork:
         w2 = rng.normal(0.0, math.sqrt(1.0 / 128.0), size=128)
         master = np.concatenate([w1.reshape(-1), b1, w2, [0.0]]).astype(np.float32)
         self.master = torch.from_numpy(master).to(dev)
-        self.params = torch.from_numpy(to_bf16(master).view(np.int16)).to(dev)
+        self.params = torch.from_numpy(network_bf16(master).view(np.int16)).to(dev)
         self.m = torch.zeros(PARAMS, dtype=torch.float32, device=dev)
         self.v = torch.zeros(PARAMS, dtype=torch.float32, device=dev)
         self.step = 0
@@ -165,16 +217,20 @@ class ProposalNetwork:
         return table
 
 
+LOG_FIELDS = 8
+
+
 class ProposalLog:
-    """Device log of proposals (bad_u, bad_v, energy, slot, delta, accepted),
-    filled by uniform-proposal walks while attached to chains.  Every replica
+    """Device log of proposals (bad_u, bad_v, energy, slot, delta, accepted,
+    u | v << 16, column u | column v << 16), filled by uniform-proposal walks
+    while attached to chains.  Every replica
     reserves one entry per proposal of a sweep; entries of invalid proposals
     keep the fill value -1, and sweeps whose reservation does not fit are not
     logged."""
 
     def __init__(self, capacity: int, device: torch.device):
         self.capacity = int(capacity)
-        self.entries = torch.full((self.capacity, 6), -1, dtype=torch.int32, device=device)
+        self.entries = torch.full((self.capacity, LOG_FIELDS), -1, dtype=torch.int32, device=device)
         self.count = torch.zeros(1, dtype=torch.int64, device=device)
 
     def attach(self, chains: TemperingChains) -> None:
@@ -196,7 +252,7 @@ class ProposalLog:
 
 
 def training_tensors(entries: torch.Tensor, temperatures: np.ndarray, device, *, label: str = "metropolis",
-                     temperature_floor: float = 0.25) -> tuple:
+                     temperature_floor: float = 0.25, features: str = "table") -> tuple:
     """Feature rows (bf16 bits) of the two moved points and their labels,
     computed on the device.
 
@@ -209,9 +265,19 @@ def training_tensors(entries: torch.Tensor, temperatures: np.ndarray, device, *,
     e = entries.to(device)
     temps = torch.from_numpy(np.ascontiguousarray(temperatures, dtype=np.float32)).to(device)
     slot = e[:, 3].long()
-    lt = torch.log(temps)[slot]
-    xu = point_features_device(e[:, 0], lt, e[:, 2]).to(torch.bfloat16).view(torch.int16).contiguous()
-    xv = point_features_device(e[:, 1], lt, e[:, 2]).to(torch.bfloat16).view(torch.int16).contiguous()
+    if features == "table":
+        lt = torch.log(temps)[slot]
+        xu = point_features_device(e[:, 0], lt, e[:, 2]).to(torch.bfloat16).view(torch.int16).contiguous()
+        xv = point_features_device(e[:, 1], lt, e[:, 2]).to(torch.bfloat16).view(torch.int16).contiguous()
+    elif features == "point":
+        # the moved slots and their columns before the move (log fields 6, 7)
+        lt_q = torch.from_numpy(temperature_codes(temperatures)).to(device)[slot]
+        u, v = e[:, 6] & 0xFFFF, e[:, 6] >> 16
+        cu, cv = e[:, 7] & 0xFFFF, e[:, 7] >> 16
+        xu = point_codes_device(e[:, 0], u >> 1, cu, lt_q, e[:, 2])
+        xv = point_codes_device(e[:, 1], v >> 1, cv, lt_q, e[:, 2])
+    else:
+        raise ValueError(f"unknown features {features!r}")
     if label == "accepted":
         y = e[:, 5].to(torch.float32)
     elif label == "metropolis":
@@ -277,6 +343,86 @@ def run_learned(chains: TemperingChains, proposals: LearnedProposals, sweeps: in
         ctypes.byref(c), proposals.weights.data_ptr(), proposals.temp_index.data_ptr(), proposals.max_energy,
         proposals.thr64.data_ptr(), proposals.k64, int(sweeps), int(steps), ladder if ladder > 1 else 0,
         chains.seed, chains._stream()))
+
+
+@dataclass
+class PointProposals:
+    """Tables the point-network walk reads for one set of chains and one
+    network (whose bf16 parameters it reads in place)."""
+
+    params: torch.Tensor           # the network's bf16 parameters (int16 bits)
+    consts: torch.Tensor           # float32 [260]: b1, w2 in MFMA-lane order, b2
+    expw: torch.Tensor             # int32 [737]
+    lt_q: torch.Tensor             # int32 [R]
+    thr64: torch.Tensor            # int64 [R][2 k64]
+    k64: int
+
+
+_EXPW: np.ndarray | None = None
+
+
+def point_exp_table() -> np.ndarray:
+    global _EXPW
+    if _EXPW is None:
+        out = np.empty(737, dtype=np.uint32)
+        _lib.check(_lib.lib().n3_point_exp_table(out.ctypes.data))
+        _EXPW = out
+    return _EXPW
+
+
+def point_constants(params_bf16: np.ndarray) -> np.ndarray:
+    """The walk's float constants of a bf16 parameter vector (ValueError if
+    W1 or b1 is off the 1/128 grid)."""
+    p = np.ascontiguousarray(params_bf16).view(np.uint16)
+    out = np.empty(260, dtype=np.float32)
+    _lib.check(_lib.lib().n3_point_constants(p.ctypes.data, out.ctypes.data))
+    return out
+
+
+def prepare_point(chains: TemperingChains, network: ProposalNetwork) -> PointProposals:
+    dev = chains.device
+    consts = torch.from_numpy(point_constants(network.params.cpu().numpy())).to(dev)
+    thr = thresholds64(chains.temperatures_host, chains.k_move)
+    return PointProposals(network.params, consts, torch.from_numpy(point_exp_table().view(np.int32)).to(dev),
+                          torch.from_numpy(temperature_codes(chains.temperatures_host)).to(dev),
+                          torch.from_numpy(thr.view(np.int64)).to(dev), chains.k_move)
+
+
+def refresh_point(proposals: PointProposals, network: ProposalNetwork) -> PointProposals:
+    """Rewrites the constants in place from the network's current parameters
+    (captured graphs stay valid: every address is unchanged)."""
+    if network.params.data_ptr() != proposals.params.data_ptr():
+        raise ValueError("the proposals were prepared for another network")
+    proposals.consts.copy_(torch.from_numpy(point_constants(network.params.cpu().numpy())))
+    return proposals
+
+
+def point_weights_device(proposals: PointProposals, columns: torch.Tensor, badness: torch.Tensor,
+                         scores: torch.Tensor, lt_q: torch.Tensor) -> torch.Tensor:
+    """The point weight of every slot of the given states (int32 [count][2n])."""
+    count, m = int(columns.shape[0]), int(columns.shape[1])
+    out = torch.empty((count, m), dtype=torch.int32, device=columns.device)
+    _lib.check(_lib.lib().n3_point_weights(columns.data_ptr(), badness.data_ptr(), scores.data_ptr(),
+                                           lt_q.data_ptr(), m // 2, count, proposals.params.data_ptr(),
+                                           proposals.consts.data_ptr(), proposals.expw.data_ptr(), out.data_ptr(),
+                                           _lib.stream_handle(columns.device)))
+    return out
+
+
+def run_learned_point(chains: TemperingChains, proposals: PointProposals, sweeps: int, steps: int,
+                      ladder_size: int = 0) -> None:
+    """`sweeps` x (point-proposal walk; exchange), graph-captured, enqueued."""
+    if steps < 1 or sweeps < 0:
+        raise ValueError("steps must be positive and sweeps non-negative")
+    ladder = int(ladder_size)
+    if ladder > 1 and len(chains.temperatures_host) % ladder:
+        raise ValueError("ladder_size must divide replica count")
+    chains._check_umbrella()
+    c = chains._chains(ladder if ladder > 1 else None)
+    _lib.check(_lib.lib().n3_temper_run_learned_point(
+        ctypes.byref(c), proposals.params.data_ptr(), proposals.consts.data_ptr(), proposals.expw.data_ptr(),
+        proposals.lt_q.data_ptr(), proposals.thr64.data_ptr(), proposals.k64, int(sweeps), int(steps),
+        ladder if ladder > 1 else 0, chains.seed, chains._stream()))
 
 
 def threshold_times(trace: list[tuple[float, int]], thresholds) -> dict:

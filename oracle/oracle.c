@@ -22,6 +22,9 @@
  *   n3o_advance_learned          the walk above with learned pair proposals
  *                                and the Metropolis-Hastings test (no reference
  *                                counterpart; the rule is stated at the function)
+ *   n3o_point_weights,           the point network's exact integer weights and
+ *   n3o_advance_point            the walk with point-network proposals and the
+ *                                delayed (two-stage) Metropolis-Hastings test
  *
  * Deliberate, documented differences from the reference (the same in the
  * product and in this oracle, so parity is exact):
@@ -752,7 +755,8 @@ void n3o_metropolis_thresholds64(const float* temperatures, int replicas, int k,
  * tests that show a detailed-balance check catches them (0 = the rule as
  * specified; 1 = S1 and S1' swapped in the acceptance test; 2 = the uniform
  * term dropped from N and N'; 3 = the moved state's weights read at the old
- * energy).  Test infrastructure only. */
+ * energy; 4 = the point walk's second, proposal-ratio stage skipped).
+ * Test infrastructure only. */
 static int learned_variant = 0;
 void n3o_set_learned_variant(int variant) { learned_variant = variant; }
 
@@ -918,6 +922,246 @@ int64_t n3o_advance_learned(int n, int replicas, int steps, uint64_t seed, uint3
                                    best_scores + r, accepted + r, reference, removed + r,
                                    weights + (int64_t)temp_index[r] * (max_energy + 1) * 64, max_energy,
                                    thr64 + (int64_t)r * 2 * k64, k64);
+    }
+    return valid;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Learned proposals from the point network (row/column features), with the  */
+/* delayed (two-stage) Metropolis-Hastings test.  No reference counterpart;   */
+/* the nearest heuristic is bad_bias (tempering.py:51-55).                    */
+/* ------------------------------------------------------------------------- */
+
+#define TAG_HAST 0x48415354u /* "HAST" */
+#define POINT_QMIN (-384)
+#define POINT_QMAX 352
+#define MLP_HIDDEN 128
+#define MLP_B1 1024
+#define MLP_W2 1152
+#define MLP_B2 1280
+
+static float bf16_value(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, sizeof f);
+    return f;
+}
+
+/* Integer proposal weight of a point with collinear-pair count b at (row,
+ * col), temperature code lt_q = round(32 ln T), state energy e.  Features
+ * are the codes k / 128 of
+ *   (4 min(b,63), 32 min(b,7), 128 [b>0], row, col, lt_q, min(e,510)/2, 2 min(e,63)).
+ * W1 and b1 lie on the grid of multiples of 1/128, so every hidden
+ * pre-activation b1 + W1 x is a multiple of 2^-14 below 2^5 in magnitude and
+ * its double sum here is exact (as is the device's fp32 MFMA sum).  The
+ * logit is two fp32 fused multiply-add chains over the ReLU outputs, hidden
+ * units 32 t + 8 (r / 4) + 4 h + r % 4 for h = 0 (from b2) and h = 1 (from
+ * 0), t = 0..3, r = 0..15 in that order, added together; the weight is
+ * expw[clamp(floor(64 g), -384, 352) + 384] (g first clamped to [-7, 6],
+ * which changes no weight). */
+static uint32_t point_weight(const uint16_t* params, const uint32_t* expw, int b, int row, int col, int lt_q, int e)
+{
+    const int bb = b < 63 ? b : 63, b7 = b < 7 ? b : 7;
+    const int code[8] = {4 * bb, 32 * b7, b > 0 ? 128 : 0, row, col, lt_q, (e < 510 ? e : 510) >> 1,
+                         2 * (e < 63 ? e : 63)};
+    float hidden[MLP_HIDDEN];
+    for (int i = 0; i < MLP_HIDDEN; ++i) {
+        double acc = bf16_value(params[MLP_B1 + i]);
+        for (int k = 0; k < 8; ++k) acc += (double)bf16_value(params[i * 8 + k]) * ((double)code[k] / 128.0);
+        hidden[i] = (float)acc;
+    }
+    float g0 = bf16_value(params[MLP_B2]), g1 = 0.0f;
+    for (int t = 0; t < 4; ++t)
+        for (int r = 0; r < 16; ++r) {
+            const int i = 32 * t + 8 * (r >> 2) + (r & 3);
+            g0 = fmaf(bf16_value(params[MLP_W2 + i]), fmaxf(hidden[i], 0.0f), g0);
+            g1 = fmaf(bf16_value(params[MLP_W2 + i + 4]), fmaxf(hidden[i + 4], 0.0f), g1);
+        }
+    const float g = fminf(fmaxf(g0 + g1, -7.0f), 6.0f);
+    int q = (int)floorf(g * 64.0f);
+    q = q < POINT_QMIN ? POINT_QMIN : (q > POINT_QMAX ? POINT_QMAX : q);
+    return expw[q - POINT_QMIN];
+}
+
+static void point_weights_state(int m, const int32_t* col, const int32_t* bad, int energy, int lt_q,
+                                const uint16_t* params, const uint32_t* expw, uint32_t* w)
+{
+    for (int k = 0; k < m; ++k) w[k] = point_weight(params, expw, bad[k], k >> 1, col[k], lt_q, energy);
+}
+
+void n3o_point_weights(int n, int64_t count, const int32_t* columns, const int32_t* badness, const int32_t* scores,
+                       const int32_t* lt_q, const uint16_t* params, const uint32_t* expw, uint32_t* out)
+{
+    const int m = 2 * n;
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < count; ++s)
+        point_weights_state(m, columns + s * m, badness + s * m, scores[s], lt_q[s], params, expw, out + s * m);
+}
+
+/* The exact energy change of swapping the columns of slots u and v, with the
+ * per-slot badness changes db[] (the pair loops of walk_slot_learned). */
+static int32_t switch_delta(int m, const int32_t* col, int u, int v, int32_t* db)
+{
+    for (int i = 0; i < m; ++i) db[i] = 0;
+    int32_t delta = 0;
+    const int ru = u >> 1, rv = v >> 1, cu = col[u], cv = col[v];
+    for (int i = 0; i < m - 1; ++i) {
+        if (i == u || i == v) continue;
+        const int ri = i >> 1, ci = col[i];
+        for (int j = i + 1; j < m; ++j) {
+            if (j == u || j == v) continue;
+            const int rj = j >> 1, cj = col[j];
+            int d = det0(ru, cv, ri, ci, rj, cj) - det0(ru, cu, ri, ci, rj, cj);
+            if (d) {
+                delta += d;
+                db[u] += d;
+                db[i] += d;
+                db[j] += d;
+            }
+            d = det0(rv, cu, ri, ci, rj, cj) - det0(rv, cv, ri, ci, rj, cj);
+            if (d) {
+                delta += d;
+                db[v] += d;
+                db[i] += d;
+                db[j] += d;
+            }
+        }
+    }
+    for (int k = 0; k < m; ++k) {
+        if (k == u || k == v) continue;
+        const int d = det0(ru, cv, rv, cu, k >> 1, col[k]) - det0(ru, cu, rv, cv, k >> 1, col[k]);
+        if (d) {
+            delta += d;
+            db[u] += d;
+            db[v] += d;
+            db[k] += d;
+        }
+    }
+    return delta;
+}
+
+/* The first slot whose inclusive prefix of w exceeds target. */
+static int inverse_cdf(int m, const uint32_t* w, uint64_t target)
+{
+    uint64_t acc = 0;
+    for (int i = 0; i < m; ++i) {
+        acc += w[i];
+        if (acc > target) return i;
+    }
+    return m - 1;
+}
+
+/* One replica's `steps` proposals with point-network weights w (recomputed
+ * for every state the walk visits).  Philox words r = (step, sweep, slot,
+ * "LRND") and h = (step, sweep, slot, "HAST"):
+ *   - with r0 < floor(0.2 * 2^32) both slots are uniform, u = mulhi(r1, m),
+ *     v = mulhi(r2, m); otherwise each slot is drawn in proportion to w by
+ *     the inverse CDF at the 64-bit targets floor(((h0 << 32) | r1) S1 / 2^64)
+ *     and floor(((h1 << 32) | r2) S1 / 2^64);
+ *   - stage 1 (Metropolis on the exact delta): continue only if
+ *     r3 < thr64(delta) = floor(2^32 exp(-delta/T));
+ *   - stage 2 (proposal ratio): accept iff h2 N S1'^2 < 2^32 N' S1^2 with
+ *     N = S1^2 + 4 m^2 w(u) w(v) and primes for the moved state.
+ * The unordered pair's probability is q = 0.4 N / (m^2 S1^2); the move is
+ * accepted with min(1, exp(-delta/T)) min(1, q'/q), a valid
+ * Metropolis-Hastings kernel (delayed acceptance: pi q min(1, pi'/pi)
+ * min(1, q'/q) = min(pi, pi') min(q, q') is symmetric). */
+static int64_t walk_slot_point(int n, int steps, uint64_t seed, uint32_t sweep, uint32_t slot, int32_t* col,
+                               int32_t* bad, int32_t* score_io, int32_t* best_col, int32_t* best_score,
+                               int32_t* accepted, const int32_t* reference, int32_t* removed_io,
+                               const uint16_t* params, const uint32_t* expw, int lt_q, const uint64_t* thr64, int k64)
+{
+    const int m = 2 * n;
+    int32_t score = *score_io, removed = *removed_io;
+    int32_t db[N3O_MAXM], nb[N3O_MAXM], col2[N3O_MAXM];
+    uint32_t w[N3O_MAXM], w2[N3O_MAXM];
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int64_t valid_count = 0;
+    point_weights_state(m, col, bad, score, lt_q, params, expw, w);
+    uint64_t s1 = 0;
+    for (int i = 0; i < m; ++i) s1 += w[i];
+    for (int step = 0; step < steps; ++step) {
+        const uint32_t ctr[4] = {(uint32_t)step, sweep, slot, TAG_LEARN}, ctr_h[4] = {(uint32_t)step, sweep, slot, TAG_HAST};
+        uint32_t r[4], h[4];
+        n3o_philox4x32_10(ctr, key, r);
+        n3o_philox4x32_10(ctr_h, key, h);
+        int u, v;
+        if (r[0] < 858993459u) {
+            u = (int)mulhi32(r[1], (uint32_t)m);
+            v = (int)mulhi32(r[2], (uint32_t)m);
+        } else {
+            const uint64_t xu = ((uint64_t)h[0] << 32) | r[1], xv = ((uint64_t)h[1] << 32) | r[2];
+            u = inverse_cdf(m, w, (uint64_t)(((unsigned __int128)xu * s1) >> 64));
+            v = inverse_cdf(m, w, (uint64_t)(((unsigned __int128)xv * s1) >> 64));
+        }
+        const int32_t cu = col[u], cv = col[v];
+        const int valid = score > 0 && (u >> 1) != (v >> 1) && cu != cv && col[u ^ 1] != cv && col[v ^ 1] != cu;
+        if (!valid) continue;
+        valid_count += 1;
+        const int ur = (u >> 1) * 2, vr = (v >> 1) * 2;
+        const int dd = (cu == reference[ur] || cu == reference[ur + 1]) + (cv == reference[vr] || cv == reference[vr + 1]) -
+                       (cv == reference[ur] || cv == reference[ur + 1]) - (cu == reference[vr] || cu == reference[vr + 1]);
+        const int32_t delta = switch_delta(m, col, u, v, db);
+        const int d = delta < -k64 ? -k64 : (delta > k64 - 1 ? k64 - 1 : delta);
+        if (!((uint64_t)r[3] < thr64[d + k64])) continue;
+        for (int i = 0; i < m; ++i) {
+            nb[i] = bad[i] + db[i];
+            col2[i] = col[i];
+        }
+        col2[u] = cv;
+        col2[v] = cu;
+        point_weights_state(m, col2, nb, learned_variant == 3 ? score : score + delta, lt_q, params, expw, w2);
+        uint64_t s1n = 0;
+        for (int i = 0; i < m; ++i) s1n += w2[i];
+        const uint64_t mm4 = 4ull * (uint64_t)m * (uint64_t)m;
+        const uint64_t uniform_sq = learned_variant == 2 ? 0 : 1;
+        const uint64_t big_n = uniform_sq * s1 * s1 + mm4 * w[u] * w[v];
+        const uint64_t big_n2 = uniform_sq * s1n * s1n + mm4 * w2[u] * w2[v];
+        uint64_t old_sq = s1 * s1, new_sq = s1n * s1n;
+        if (learned_variant == 1) {
+            const uint64_t t = old_sq;
+            old_sq = new_sq;
+            new_sq = t;
+        }
+        const unsigned __int128 lhs = (unsigned __int128)h[2] * ((unsigned __int128)big_n * new_sq);
+        const unsigned __int128 rhs = ((unsigned __int128)big_n2 * old_sq) << 32;
+        if (learned_variant != 4 && !(lhs < rhs)) continue;
+        col[u] = cv;
+        col[v] = cu;
+        score += delta;
+        removed += dd;
+        *accepted += 1;
+        for (int i = 0; i < m; ++i) {
+            bad[i] = nb[i];
+            w[i] = w2[i];
+        }
+        s1 = s1n;
+        if (score < *best_score) {
+            *best_score = score;
+            for (int i = 0; i < m; ++i) best_col[i] = col[i];
+        }
+    }
+    *score_io = score;
+    *removed_io = removed;
+    return valid_count;
+}
+
+int64_t n3o_advance_point(int n, int replicas, int steps, uint64_t seed, uint32_t sweep, int32_t* columns,
+                          int32_t* badness, int32_t* scores, int32_t* best_columns, int32_t* best_scores,
+                          int32_t* accepted, const int32_t* reference, int32_t* removed, const uint16_t* params,
+                          const uint32_t* expw, const int32_t* lt_q, const uint64_t* thr64, int k64, int threads)
+{
+    const int m = 2 * n;
+    int64_t valid = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : valid)
+    for (int r = 0; r < replicas; ++r) {
+        valid += walk_slot_point(n, steps, seed, sweep, (uint32_t)r, columns + (int64_t)r * m, badness + (int64_t)r * m,
+                                 scores + r, best_columns + (int64_t)r * m, best_scores + r, accepted + r, reference,
+                                 removed + r, params, expw, lt_q[r], thr64 + (int64_t)r * 2 * k64, k64);
     }
     return valid;
 }

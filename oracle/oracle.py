@@ -79,6 +79,15 @@ def lib():
             ctypes.c_int, ctypes.c_int,
         ]
         L.n3o_advance_learned.restype = ctypes.c_int64
+        _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+        L.n3o_point_weights.argtypes = [ctypes.c_int, ctypes.c_int64, _i32p, _i32p, _i32p, _i32p, _u16p, _u32p, _u32p]
+        L.n3o_point_weights.restype = None
+        L.n3o_advance_point.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+            _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _u16p, _u32p, _i32p, _u64p,
+            ctypes.c_int, ctypes.c_int,
+        ]
+        L.n3o_advance_point.restype = ctypes.c_int64
         L.n3o_set_learned_variant.argtypes = [ctypes.c_int]
         L.n3o_set_learned_variant.restype = None
         _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
@@ -281,6 +290,18 @@ class TemperingReplay:
         self.sweep += 1
         return int(valid)
 
+    def advance_point(self, steps: int, params, expw, lt_q, thr64, threads: int = 0) -> int:
+        """The walk with point-network proposals and the two-stage
+        Metropolis-Hastings test (see n3o_advance_point) on the same state."""
+        thr64 = np.ascontiguousarray(thr64, dtype=np.uint64)
+        valid = lib().n3o_advance_point(
+            self.n, self.replicas, steps, self.seed, self.sweep, self.columns, self.badness, self.scores,
+            self.best_columns, self.best_scores, self.accepted, self.reference, self.removed_points,
+            np.ascontiguousarray(params, dtype=np.uint16), np.ascontiguousarray(expw, dtype=np.uint32),
+            np.ascontiguousarray(lt_q, dtype=np.int32), thr64, thr64.shape[1] // 2, threads)
+        self.sweep += 1
+        return int(valid)
+
     def exchange(self, ladder_size: int) -> int:
         if np.any(self.strengths != 0):
             raise ValueError("temperature-only exchange is disabled with umbrella potentials")
@@ -289,6 +310,18 @@ class TemperingReplay:
                                    self.badness, self.scores, self.removed_points, thr, sign, self.k_exchange)
         self.round += 1
         return int(count)
+
+
+def point_weights(columns, badness, scores, lt_q, params, expw) -> np.ndarray:
+    """The point network's integer weight of every slot of every state
+    (n3o_point_weights): columns and badness int32[count][2n]."""
+    columns = _i32(columns)
+    count, m = columns.shape
+    out = np.empty((count, m), dtype=np.uint32)
+    lib().n3o_point_weights(m // 2, count, columns, _i32(badness), _i32(scores), _i32(lt_q),
+                            np.ascontiguousarray(params, dtype=np.uint16), np.ascontiguousarray(expw, dtype=np.uint32),
+                            out)
+    return out
 
 
 def set_learned_variant(variant: int) -> None:

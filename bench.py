@@ -248,26 +248,36 @@ def batched_cpu_baseline(pkg, oracle, replicas_per_size: int = 256) -> dict:
             "per_size_proposals_per_s": per_size}
 
 
+AB_SIDES = (("uniform", "uniform"), ("learned", "learned_table"), ("learned_point", "learned_point"))
+
+
 def _ab_summary(res: dict) -> dict:
     out = {"budget_seconds_each": res["config"]["seconds"]}
-    for side in ("uniform", "learned"):
-        r = res[side]
-        out[side] = {"sweeps": r["sweeps"], "best_E": min(e for _, e in r["trace"]),
-                     "seconds_to_E": r["threshold_seconds"]}
-    out["learned"]["training_tuples"] = res["learned"]["training_tuples"]
-    out["learned"]["retrains"] = res["learned"]["retrains"]
-    out["learned"]["seconds_before_learned_proposals"] = res["learned"]["seconds_until_learned_proposals"]
-    out["time_to_E12"] = {"uniform": res["uniform"]["threshold_seconds"]["12"],
-                          "learned": res["learned"]["threshold_seconds"]["12"]}
+    for key, name in AB_SIDES:
+        if key not in res:
+            continue
+        r = res[key]
+        side = {"sweeps": r["sweeps"], "best_E": min(e for _, e in r["trace"]), "seconds_to_E": r["threshold_seconds"]}
+        if key != "uniform":
+            side.update({"training_tuples": r["training_tuples"], "retrains": r["retrains"],
+                         "seconds_before_learned_proposals": r["seconds_until_learned_proposals"]})
+        out[name] = side
+    out["time_to_E12"] = {name: res[key]["threshold_seconds"].get("12") for key, name in AB_SIDES if key in res}
     return out
+
+
+AB_THRESHOLDS = (80, 70, 60, 55, 50, 47, 45, 44, 43, 42, 41, 40, 38, 35, 30, 12)
 
 
 def learned_ab(pkg, seconds: float) -> dict:
     """n=75, 4096 replicas as 64 ladders x 64 temperatures from seeded random
-    states: uniform against learned proposals, same seeds, replicas and GPU,
-    equal wall-clock budget (the learned side pays for its plain warm-up,
-    logging and online training inside its budget)."""
-    res = pkg.learned.ab_run(75, 4096, 64, seconds=seconds, seed=11, state_seed=1, train_sweeps=300, chunk=100)
+    states: uniform proposals against both learned modes (the tabulated
+    count weights and the point network with row/column features), same
+    seeds, replicas and GPU, equal wall-clock budget; each learned side pays
+    for its plain warm-up, logging and online training inside its budget.
+    The config-5 result is the time to each energy level."""
+    res = pkg.learned.ab_run(75, 4096, 64, seconds=seconds, seed=11, state_seed=1, train_sweeps=300, chunk=100,
+                             thresholds=AB_THRESHOLDS, modes=("table", "point"))
     out = _ab_summary(res)
     out["start"] = "seeded random states (state seed 1)"
     return out
@@ -279,10 +289,10 @@ NEAR_SOLUTION = ROOT / "tests" / "golden" / "best-near-solution-E10.json"
 def learned_ab_near_solution(pkg, seconds: float, moves: int = 12, seeds=(12, 13, 14)) -> dict | None:
     """The A/B from the reference's saved E=10 near-solution, each replica
     perturbed by `moves` random legal 2-switches (seeded), where E <= 12 is
-    reachable within the budget so time-to-E<=12 is measured.  The network is
-    trained beforehand on the logs of a plain calibration run from another
-    perturbation (its time is reported, not counted); online retraining
-    still runs inside the learned side's budget.  One race per seed."""
+    reachable within the budget so time-to-E<=12 is measured.  Each learned
+    side trains its network inside its own budget: 60 plain sweeps from the
+    race's start, the last 2 logged, 300 Adam steps, then learned proposals
+    with online retraining.  One race per seed."""
     import numpy as np
 
     if not NEAR_SOLUTION.exists():
@@ -290,28 +300,30 @@ def learned_ab_near_solution(pkg, seconds: float, moves: int = 12, seeds=(12, 13
     _, points, _ = pkg.io.load_candidate(NEAR_SOLUTION, 75)
     state = pkg.points_to_permutations(points, 75)
     base = np.repeat(np.stack((np.array(state.P), np.array(state.Q)), 1).reshape(1, 150).astype(np.int32), 4096, 0)
-    net, calibration = pkg.learned.pretrain(75, 4096, 64, seed=11, start=pkg.random_two_switches(base, moves, seed=99))
     races = []
     for s in seeds:
-        res = pkg.learned.ab_run(75, 4096, 64, seconds=seconds, seed=11, state_seed=1, chunk=50,
-                                 thresholds=(30, 20, 15, 12, 11, 10), start=pkg.random_two_switches(base, moves, seed=s),
-                                 network=net.clone())
+        res = pkg.learned.ab_run(75, 4096, 64, seconds=seconds, seed=11, state_seed=1, chunk=20, train_sweeps=60,
+                                 train_steps=300, thresholds=(30, 20, 15, 12, 11, 10),
+                                 start=pkg.random_two_switches(base, moves, seed=s), modes=("table", "point"))
         summary = _ab_summary(res)
         summary["perturbation_seed"] = s
         races.append(summary)
-    t_u = [r["time_to_E12"]["uniform"] for r in races]
-    t_l = [r["time_to_E12"]["learned"] for r in races]
 
     def median(values):
         values = sorted(v if v is not None else float("inf") for v in values)
         mid = values[len(values) // 2]
         return None if mid == float("inf") else mid
 
-    return {"start": f"reference near-solution (E=10) + {moves} random 2-switches per replica",
-            "budget_seconds_each": seconds, "calibration": {k: calibration[k] for k in ("seconds", "tuples", "sweeps",
-                                                                                     "steps")},
-            "races": races, "time_to_E12_median": {"uniform": median(t_u), "learned": median(t_l)},
-            "learned_first": sum(1 for a, b in zip(t_u, t_l) if b is not None and (a is None or b < a))}
+    names = [name for _, name in AB_SIDES]
+    out = {"start": f"reference near-solution (E=10) + {moves} random 2-switches per replica",
+           "budget_seconds_each": seconds, "training": "inside each learned side's budget",
+           "races": races,
+           "time_to_E12_median": {name: median([r["time_to_E12"][name] for r in races]) for name in names}}
+    for name in names[1:]:
+        out[f"{name}_first"] = sum(1 for r in races if r["time_to_E12"][name] is not None and
+                                   (r["time_to_E12"]["uniform"] is None or
+                                    r["time_to_E12"][name] < r["time_to_E12"]["uniform"]))
+    return out
 
 
 def host_array_rates(pkg, torch, p, q, repeats: int = 5) -> dict:

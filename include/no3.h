@@ -74,7 +74,10 @@ int n3_score_population(const int32_t* p, const int32_t* q, int n, int64_t count
 
 /* Synchronises `stream` and reports whether the scoring launches since the
  * last call met a column outside [0, n) (N3_EINVAL; the affected scores are
- * -1).  The reference has no range check (gpu.py:61-66 checks shapes only). */
+ * -1).  The flag is one word per device, shared by every stream and thread:
+ * a launch on another stream that has not finished is not covered, and an
+ * error it sets may be reported here.  The reference has no range check
+ * (gpu.py:61-66 checks shapes only). */
 int n3_score_status(void* stream);
 
 /* Score after every 2-switch of slot u with slot v of ONE state, int32 with

@@ -10,6 +10,8 @@
 // bad' the sum of C(cnt, 2) over the direction histogram seen from u' (v').
 // One wavefront evaluates one (state, u, v) at a time in O(m) table lookups
 // instead of the reference's O(m^2) determinant pair loops.
+#include <mutex>
+
 #include "runtime.h"
 #include "wave_ops.h"
 
@@ -373,9 +375,11 @@ static int run_switch(SwitchArgs args, bool wide, hipStream_t stream) {
 
 int switch_status_word(int** word) {
   static int* words[64] = {nullptr};
+  static std::mutex mutex;
   int device = 0;
   (void)hipGetDevice(&device);
   if (device < 0 || device >= 64) return set_error(N3_ENODEV, "device index %d out of range", device);
+  std::lock_guard<std::mutex> lock(mutex);
   if (!words[device]) {
     const int st = hip_status(hipMalloc(&words[device], sizeof(int)), "hipMalloc(status)");
     if (st) return st;

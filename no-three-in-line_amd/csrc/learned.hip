@@ -839,7 +839,12 @@ struct LearnedEntry {
   hipGraph_t graph;
   hipGraphExec_t exec;
   int per_graph;
+  hipStream_t stream;
 };
+
+// At most this many cached learned-walk graphs; the oldest is dropped (once
+// its stream has drained) to make room.
+constexpr size_t kMaxLearnedGraphs = 16;
 static std::mutex g_learned_mutex;
 static std::vector<LearnedEntry> g_learned;
 
@@ -867,6 +872,7 @@ static int run_graphs(const n3_chains* chains, const void* args, size_t args_siz
     LearnedEntry e;
     e.key = key;
     e.per_graph = per_graph;
+    e.stream = s;
     hipStream_t cap = nullptr;
     st = hip_status(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
     if (st) return st;
@@ -886,6 +892,13 @@ static int run_graphs(const n3_chains* chains, const void* args, size_t args_siz
     }
     (void)hipStreamDestroy(cap);
     if (st) return st;
+    if (g_learned.size() >= kMaxLearnedGraphs) {
+      LearnedEntry& old = g_learned.front();
+      (void)hipStreamSynchronize(old.stream);
+      (void)hipGraphExecDestroy(old.exec);
+      (void)hipGraphDestroy(old.graph);
+      g_learned.erase(g_learned.begin());
+    }
     g_learned.push_back(e);
     entry = &g_learned.back();
   }

@@ -788,12 +788,14 @@ static int launch_determinant(const int32_t* p, const int32_t* q, int n, int64_t
 // Per-device status word, cleared before and read after a scoring launch.
 static int* status_word(int* st_out) {
   static int* words[64] = {nullptr};
+  static std::mutex mutex;
   int device = 0;
   (void)hipGetDevice(&device);
   if (device < 0 || device >= 64) {
     *st_out = set_error(N3_ENODEV, "device index %d out of range", device);
     return nullptr;
   }
+  std::lock_guard<std::mutex> lock(mutex);
   if (!words[device]) {
     int* w = nullptr;
     *st_out = hip_status(hipMalloc(&w, sizeof(int)), "hipMalloc(status)");

@@ -698,6 +698,20 @@ struct GraphEntry {
 static std::mutex g_graph_mutex;
 static std::vector<GraphEntry> g_graphs;
 
+// Cached graphs per kind: the oldest is dropped (after its stream has
+// drained, since a replay may still be using it) when a new one would
+// exceed the cap, so long drivers do not accumulate graphs.
+constexpr size_t kMaxCachedGraphs = 16;
+
+static void drop_oldest_graph() {
+  GraphEntry& e = g_graphs.front();
+  (void)hipStreamSynchronize(e.key.stream);
+  (void)hipGraphExecDestroy(e.exec);
+  (void)hipGraphDestroy(e.graph);
+  (void)hipFree(e.done);
+  g_graphs.erase(g_graphs.begin());
+}
+
 // A graph of `sweeps` x (walk, exchange): the exchange kernel advances both
 // counters through the entry's `done` word, so a sweep is two launches.
 static int capture_graph(const n3_chains* c, const WalkPlan& plan, int sweeps, int steps, int ladder, int bad_bias,
@@ -829,9 +843,19 @@ struct BatchEntry {
   hipGraph_t graph;
   hipGraphExec_t exec;
   int sweeps_per_graph;
+  hipStream_t stream;
 };
 
 static std::vector<BatchEntry> g_batches;
+
+static void drop_oldest_batch() {
+  BatchEntry& e = g_batches.front();
+  (void)hipStreamSynchronize(e.stream);
+  (void)hipGraphExecDestroy(e.exec);
+  (void)hipGraphDestroy(e.graph);
+  (void)hipFree(e.plan.device);
+  g_batches.erase(g_batches.begin());
+}
 
 }  // namespace no3
 
@@ -870,6 +894,7 @@ extern "C" int n3_temper_batched_run(const n3_group* groups, int group_count, in
     BatchEntry e;
     e.key = key;
     e.sweeps_per_graph = per_graph;
+    e.stream = s;
     int st = batch_plan(groups, group_count, &e.plan);
     if (st) return st;
     hipStream_t cap = nullptr;
@@ -894,6 +919,7 @@ extern "C" int n3_temper_batched_run(const n3_group* groups, int group_count, in
       (void)hipFree(e.plan.device);
       return st;
     }
+    if (g_batches.size() >= kMaxCachedGraphs) drop_oldest_batch();
     g_batches.push_back(std::move(e));
     entry = &g_batches.back();
   }
@@ -964,6 +990,7 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
         (void)hipFree(done);
         return st;
       }
+      if (g_graphs.size() >= kMaxCachedGraphs) drop_oldest_graph();
       g_graphs.push_back(GraphEntry{key, exec, graph, done});
     }
   }

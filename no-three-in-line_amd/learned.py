@@ -437,22 +437,24 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
            train_sweeps: int = 200, log_sweeps: int = 2, train_steps: int = 400, retrain_seconds: float = 2.0,
            retrain_steps: int = 100, log_capacity: int = 1 << 21,
            chunk: int = 50, thresholds=(80, 70, 60, 55, 50, 45, 40, 12), tmin: float = 0.05, tmax: float = 2.0,
-           start: np.ndarray | None = None, network: "ProposalNetwork | None" = None,
+           start: np.ndarray | None = None, network: "ProposalNetwork | None" = None, modes=("table",),
            device: torch.device | None = None) -> dict:
     """Uniform vs learned proposals on the same seeds, replicas and GPU.
 
-    Both start from the same states: seeded generator states, or the rows of
-    `start` (int32[replicas][2n]) when given.  The uniform run proceeds in
-    graph-captured chunks.  The learned run first runs `train_sweeps` plain
-    sweeps, the last `log_sweeps` of them logged, trains the network on that
-    log (`train_steps` Adam steps) and continues with learned proposals; every
-    `retrain_seconds` it logs `log_sweeps` more plain sweeps and takes
-    `retrain_steps` more steps on them (online training).  With `network`
-    given (trained beforehand, e.g. by `pretrain` on another run's logs) the
-    learned run starts with learned proposals at once and only the online
-    retraining happens inside its budget.  The trace records (elapsed
-    seconds, best energy) after every chunk; logging and training time count
-    against the learned run."""
+    Every side starts from the same states: seeded generator states, or the
+    rows of `start` (int32[replicas][2n]) when given, and gets the same
+    wall-clock budget.  The uniform run proceeds in graph-captured chunks.
+    Each learned side (one per entry of `modes`: "table" for the tabulated
+    (temperature, energy, count) weights, "point" for the point network with
+    row and column features evaluated inside the walk) first runs
+    `train_sweeps` plain sweeps, the last `log_sweeps` of them logged, trains
+    a fresh network on that log (`train_steps` Adam steps) and continues
+    with learned proposals; every `retrain_seconds` it logs `log_sweeps` more
+    plain sweeps and takes `retrain_steps` more steps on them (online
+    training).  All of that counts against its budget.  With `network` given
+    (trained beforehand), a clone of it starts the learned side at once and
+    only the online retraining happens inside the budget.  The traces record
+    (elapsed seconds, best energy) after every chunk."""
     from .batched import seeded_chains
 
     dev = _lib.device(None if device is None else device.index)
@@ -461,6 +463,9 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
         raise ValueError("need 0 < log_sweeps <= train_sweeps")
     if log_sweeps * steps * replicas > log_capacity:
         raise ValueError("log_capacity is too small for log_sweeps")
+    for mode in modes:
+        if mode not in ("table", "point"):
+            raise ValueError(f"unknown mode {mode!r}")
     result = {}
 
     def chains():
@@ -485,13 +490,29 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
     uni.validate(range(0, replicas, max(1, replicas // 64)))
     result["uniform"] = {"trace": trace_u, "sweeps": chunk * len(trace_u),
                          "threshold_seconds": threshold_times(trace_u, thresholds)}
-    # learned
-    lrn = chains()
+    for mode in modes:
+        result["learned" if mode == "table" else "learned_point"] = _learned_side(
+            chains(), mode, seconds=seconds, seed=seed, steps=steps, ladder=ladder, chunk=chunk,
+            train_sweeps=train_sweeps, log_sweeps=log_sweeps, train_steps=train_steps,
+            retrain_seconds=retrain_seconds, retrain_steps=retrain_steps, log_capacity=log_capacity,
+            thresholds=thresholds, network=None if network is None else network.clone(), dev=dev)
+    result["config"] = {"n": n, "replicas": replicas, "ladder": ladder, "seconds": seconds, "seed": seed,
+                        "state_seed": state_seed if start is None else None, "tmin": tmin, "tmax": tmax,
+                        "train_sweeps": train_sweeps, "log_sweeps": log_sweeps, "train_steps": train_steps,
+                        "retrain_seconds": retrain_seconds, "retrain_steps": retrain_steps,
+                        "pretrained": network is not None, "modes": list(modes)}
+    return result
+
+
+def _learned_side(lrn: TemperingChains, mode: str, *, seconds, seed, steps, ladder, chunk, train_sweeps, log_sweeps,
+                  train_steps, retrain_seconds, retrain_steps, log_capacity, thresholds, network, dev) -> dict:
+    """One learned side of ab_run (see there) on the chains `lrn`."""
+    replicas = len(lrn.temperatures_host)
     net = ProposalNetwork(seed=seed, device=dev) if network is None else network
     log = ProposalLog(log_capacity, dev)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    trace_l = []
+    trace = []
     sweeps = 0
 
     def plain(count, logged):
@@ -509,12 +530,17 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
             plain(step, False)
             done += step
             torch.cuda.synchronize(dev)
-            trace_l.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
+            trace.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
         plain(log_sweeps, True)
-        xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev)
+        xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev, features=mode)
         tuples = int(y.shape[0])
         losses = net.train(xu, xv, y, steps=train_steps, seed=seed)
-    proposals = prepare(lrn, net)
+    if mode == "table":
+        proposals = prepare(lrn, net)
+        walk = run_learned
+    else:
+        proposals = prepare_point(lrn, net)
+        walk = run_learned_point
     torch.cuda.synchronize(dev)
     train_seconds = time.perf_counter() - t0
     last_train = train_seconds
@@ -523,32 +549,29 @@ def ab_run(n: int, replicas: int, ladder: int, *, seconds: float, seed: int, sta
         if retrain_seconds > 0 and time.perf_counter() - t0 - last_train >= retrain_seconds:
             log.reset()
             plain(log_sweeps, True)
-            xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev)
+            xu, xv, y = training_tensors(log.rows(), lrn.temperatures_host, dev, features=mode)
             tuples += int(y.shape[0])
             losses += net.train(xu, xv, y, steps=retrain_steps, seed=seed + retrains + 1)
-            proposals = refresh(proposals, lrn, net)
+            if mode == "table":
+                proposals = refresh(proposals, lrn, net)
+            else:
+                proposals = refresh_point(proposals, net)
             torch.cuda.synchronize(dev)
             last_train = time.perf_counter() - t0
             retrains += 1
-        run_learned(lrn, proposals, chunk, steps, ladder)
+        walk(lrn, proposals, chunk, steps, ladder)
         sweeps += chunk
         torch.cuda.synchronize(dev)
-        trace_l.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
+        trace.append((time.perf_counter() - t0, int(lrn.best_scores.min().item())))
     lrn.validate(range(0, replicas, max(1, replicas // 64)))
-    result["learned"] = {"trace": trace_l, "sweeps": sweeps, "training_tuples": tuples, "training_losses": losses,
-                         "retrains": retrains, "seconds_until_learned_proposals": train_seconds,
-                         "threshold_seconds": threshold_times(trace_l, thresholds)}
-    result["config"] = {"n": n, "replicas": replicas, "ladder": ladder, "seconds": seconds, "seed": seed,
-                        "state_seed": state_seed if start is None else None, "tmin": tmin, "tmax": tmax,
-                        "train_sweeps": train_sweeps, "log_sweeps": log_sweeps, "train_steps": train_steps,
-                        "retrain_seconds": retrain_seconds, "retrain_steps": retrain_steps,
-                        "pretrained": network is not None}
-    return result
+    return {"trace": trace, "sweeps": sweeps, "training_tuples": tuples, "training_losses": losses,
+            "retrains": retrains, "seconds_until_learned_proposals": train_seconds,
+            "threshold_seconds": threshold_times(trace, thresholds), "mode": mode}
 
 
 def pretrain(n: int, replicas: int, ladder: int, *, seed: int, start: np.ndarray | None = None, state_seed: int = 2,
              sweeps: int = 300, log_sweeps: int = 2, steps: int = 400, tmin: float = 0.05, tmax: float = 2.0,
-             device: torch.device | None = None) -> tuple["ProposalNetwork", dict]:
+             features: str = "table", device: torch.device | None = None) -> tuple["ProposalNetwork", dict]:
     """A network trained on the logs of a separate plain calibration run
     (`sweeps` uniform sweeps from `start` or seeded states, the last
     `log_sweeps` logged, `steps` Adam steps): returns it with the run's
@@ -567,7 +590,7 @@ def pretrain(n: int, replicas: int, ladder: int, *, seed: int, start: np.ndarray
     chains.log = log
     chains.run(log_sweeps, 2 * n, ladder)
     chains.log = None
-    xu, xv, y = training_tensors(log.rows(), chains.temperatures_host, dev)
+    xu, xv, y = training_tensors(log.rows(), chains.temperatures_host, dev, features=features)
     net = ProposalNetwork(seed=seed, device=dev)
     losses = net.train(xu, xv, y, steps=steps, seed=seed)
     torch.cuda.synchronize(dev)

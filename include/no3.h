@@ -197,7 +197,18 @@ typedef struct n3_group {
  * reference caps n at 75 and runs one size per process, tempering.py:124). */
 int n3_temper_batched_run(const n3_group* groups, int group_count, int sweeps, void* stream);
 
-/* Drops cached graphs (all of them). */
+/* The same run as n3_temper_batched_run in ONE persistent cooperative
+ * launch for all `sweeps` sweeps: one block per compute unit, each sweep a
+ * queue of work items (W_g replicas of group g, largest n first, W_g =
+ * chains.block_waves, or as many as fit in LDS when 0) drained from a
+ * device counter, a grid barrier, the exchanges of every group inside the
+ * kernel, and a second barrier.
+ * Trajectories and counters are bit-identical to n3_temper_batched_run's.
+ * No reference counterpart (config 4 of the benchmark). */
+int n3_temper_batched_persistent(const n3_group* groups, int group_count, int sweeps, void* stream);
+
+/* Drops cached graphs and persistent-run plans (all of them); synchronises
+ * the current device first. */
 void n3_temper_release(void);
 
 /* ---- learned move proposals (no reference counterpart) ----------------- */

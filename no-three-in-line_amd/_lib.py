@@ -104,6 +104,7 @@ SIGNATURES = {
     "n3_temper_exchange": (_i, [ctypes.POINTER(Chains), _i, _u64, _vp]),
     "n3_temper_run": (_i, [ctypes.POINTER(Chains), _i, _i, _i, _i, _u64, _vp]),
     "n3_temper_batched_run": (_i, [ctypes.POINTER(Group), _i, _i, _vp]),
+    "n3_temper_batched_persistent": (_i, [ctypes.POINTER(Group), _i, _i, _vp]),
     "n3_temper_release": (None, []),
     "n3_metropolis_thresholds": (_i, [_vp, _i, _i, _vp]),
     "n3_mlp_params": (_i, []),

@@ -28,27 +28,38 @@ def default_cache_path() -> Path:
 
 
 def tune_size(n: int, replicas: int, *, ladder: int = 64, sweeps: int = 4, candidates=CANDIDATES,
-              seed: int = 0) -> dict:
-    """Times `sweeps` sweeps of 2n proposals for every candidate block shape."""
+              seed: int = 0, persistent: bool = False) -> dict:
+    """Times `sweeps` sweeps of 2n proposals for every candidate block shape:
+    per-sweep walk launches, or (persistent) the one-launch batched run of
+    this size alone."""
+    from .batched import BatchGroup, run_batched
+
     dev = _lib.device()
     ladder = min(ladder, replicas)
     chains = seeded_chains(n, replicas, ladder, seed=seed, device=dev)
+
+    def run(count):
+        if persistent:
+            run_batched([BatchGroup(chains, ladder, 2 * n)], count, persistent=True)
+        else:
+            chains.run(count, 2 * n, ladder)
+
     timings = {}
     for w in candidates:
         chains.block_waves = w
         try:
-            chains.run(1, 2 * n, ladder)
+            run(1)
         except ValueError:
             continue  # does not fit in LDS
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        chains.run(sweeps, 2 * n, ladder)
+        run(sweeps)
         torch.cuda.synchronize(dev)
         elapsed = time.perf_counter() - t0
         timings[w] = replicas * sweeps * 2 * n / elapsed
     _lib.lib().n3_temper_release()
     best = max(timings, key=timings.get)
-    return {"n": n, "replicas": replicas, "block_waves": best,
+    return {"n": n, "replicas": replicas, "block_waves": best, "persistent": persistent,
             "proposals_per_s": {str(k): v for k, v in timings.items()}}
 
 
@@ -63,8 +74,9 @@ def tune(sizes: dict[int, int], *, cache: Path | None = None, refresh: bool = Fa
         except (ValueError, OSError):
             table = {}
     out = {}
+    persistent = bool(kw.get("persistent", False))
     for n, replicas in sizes.items():
-        key = f"{n}:{replicas}"
+        key = f"{n}:{replicas}" + (":persistent" if persistent else "")
         if key not in table:
             table[key] = tune_size(n, replicas, **kw)
         out[n] = int(table[key]["block_waves"])

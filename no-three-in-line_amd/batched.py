@@ -25,8 +25,10 @@ class BatchGroup:
     bad_bias: int = 0
 
 
-def run_batched(groups: list[BatchGroup], sweeps: int) -> None:
-    """Enqueues `sweeps` sweeps of every group on the current stream."""
+def run_batched(groups: list[BatchGroup], sweeps: int, persistent: bool = False) -> None:
+    """Enqueues `sweeps` sweeps of every group on the current stream: one
+    walk launch per sweep replayed from captured graphs, or (persistent) all
+    sweeps in one cooperative launch; both give the same trajectories."""
     if not 1 <= len(groups) <= _lib.MAX_GROUPS:
         raise ValueError(f"between 1 and {_lib.MAX_GROUPS} groups are supported")
     if sweeps < 0:
@@ -48,7 +50,8 @@ def run_batched(groups: list[BatchGroup], sweeps: int) -> None:
         arr[i].bad_bias = int(g.bad_bias)
         arr[i].seed = g.chains.seed
     dev = devices.pop()
-    _lib.check(_lib.lib().n3_temper_batched_run(arr, len(groups), int(sweeps), _lib.stream_handle(dev)))
+    entry = _lib.lib().n3_temper_batched_persistent if persistent else _lib.lib().n3_temper_batched_run
+    _lib.check(entry(arr, len(groups), int(sweeps), _lib.stream_handle(dev)))
 
 
 def seeded_chains(n: int, replicas: int, ladder: int, *, seed: int, tmin: float = 0.05, tmax: float = 2.0,

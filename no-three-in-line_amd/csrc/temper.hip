@@ -27,6 +27,8 @@
 #include <mutex>
 #include <vector>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "runtime.h"
 #include "temper_internal.h"
 #include "walk.h"
@@ -78,13 +80,12 @@ __device__ inline double umbrella_energy(int delta, int dd, int removed, float s
 // wavefront's LDS direction table and histogram.
 template <int CH, bool kUmbrella = false>
 __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t* hist, const DirTableInfo& info,
-                             int slot_in) {
+                             int slot_in, uint32_t sweep) {
   const int lane = lane_id();
   // the slot is the same on every lane: keep it (and the Philox counters,
   // proposals and table indexes derived from it) in scalar registers
   const int slot = __builtin_amdgcn_readfirstlane(slot_in);
   const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
-  const uint32_t sweep = a.counters[0];
   const long long base = (long long)slot * m;
 
   // per lane: keys, badness and reference columns of slots lane + 64 c
@@ -384,7 +385,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
   const int slot = blockIdx.x * waves + wave;
-  if (slot < a.replicas) walk_replica<CH, kUmbrella>(a, table, hist, info, slot);
+  if (slot < a.replicas) walk_replica<CH, kUmbrella>(a, table, hist, info, slot, a.counters[0]);
 }
 
 // Several populations of different sizes in one launch: block b belongs to
@@ -417,10 +418,10 @@ __global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __
   const int slot = ((int)blockIdx.x - G.block_begin) * G.waves + wave;
   if (slot >= G.walk.replicas) return;
   switch (G.ch) {
-    case 1: walk_replica<1>(G.walk, table, hist, info, slot); break;
-    case 2: walk_replica<2>(G.walk, table, hist, info, slot); break;
-    case 3: walk_replica<3>(G.walk, table, hist, info, slot); break;
-    default: walk_replica<4>(G.walk, table, hist, info, slot); break;
+    case 1: walk_replica<1>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 2: walk_replica<2>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 3: walk_replica<3>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    default: walk_replica<4>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
   }
 }
 
@@ -446,9 +447,8 @@ struct ExchangeArgs {
 // One wavefront per candidate pair (a, a+1); pairs start at round % 2 inside
 // every ladder and do not overlap, so the swaps need no synchronisation.
 // Returns whether the pair swapped.
-__device__ bool exchange_pair(const ExchangeArgs& x, int wave) {
+__device__ bool exchange_pair(const ExchangeArgs& x, int wave, uint32_t round) {
   const int lane = lane_id();
-  const uint32_t round = x.counters[1];
   const int per_ladder = x.ladder / 2;  // candidate pairs per ladder (upper bound)
   const int ladders = x.replicas / x.ladder;
   if (wave >= ladders * per_ladder) return false;
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(1024) exchange_kernel(ExchangeArgs x) {
   __shared__ uint32_t swaps;
   if (threadIdx.x == 0) swaps = 0;
   __syncthreads();
-  const bool swapped = exchange_pair(x, (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+  const bool swapped = exchange_pair(x, (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave), x.counters[1]);
   if (swapped && lane_id() == 0) atomicAdd(&swaps, 1u);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -524,6 +524,7 @@ struct BatchTables {
   uint32_t round_step[N3_MAX_GROUPS];     // 1 if the group exchanges
   int count;
   uint32_t done;  // blocks of the running exchange launch that finished
+  int total_items;  // walk blocks of one sweep over all groups (the persistent run's work items)
 };
 
 // As exchange_kernel, for every group of a batched run: swaps are counted per
@@ -537,7 +538,8 @@ __global__ void __launch_bounds__(1024) exchange_batched_kernel(BatchTables* __r
   int g = 0;
   while (g + 1 < t->count && wave >= t->exchange_begin[g + 1]) ++g;
   if (wave < t->exchange_begin[t->count] && t->round_step[g] != 0) {
-    if (exchange_pair(t->exchange[g], wave - t->exchange_begin[g]) && lane_id() == 0) atomicAdd(&swaps[g], 1u);
+    if (exchange_pair(t->exchange[g], wave - t->exchange_begin[g], t->exchange[g].counters[1]) && lane_id() == 0)
+      atomicAdd(&swaps[g], 1u);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -551,6 +553,95 @@ __global__ void __launch_bounds__(1024) exchange_batched_kernel(BatchTables* __r
       }
       atomicExch(&t->done, 0u);
     }
+  }
+}
+
+// All `sweeps` sweeps of a batched run in ONE cooperative launch with one
+// block per CU (the launch guarantees that every block is resident).  Each
+// sweep drains a queue of work items -- item i is walk block i of the
+// per-sweep launch: W_g replicas of group g, groups largest n first -- that
+// blocks take one at a time from a device counter, restaging the direction
+// table in LDS when the group changes; then a grid barrier, the exchanges of
+// every group (one wavefront per candidate pair, strided over the grid), and
+// a second barrier.  Sweep and round numbers are the groups' counters at
+// launch plus the sweep index; block 0 advances the counters at the end.
+// queue[2] holds the item counters of even and odd sweeps (zero at launch;
+// each is cleared for its next use during the sweep before it, and both are
+// zero again when the launch ends).  Every trajectory is bit-identical to
+// the per-sweep launches'.
+__global__ void __launch_bounds__(1024) walk_persistent_kernel(BatchTables* __restrict__ t, int sweeps,
+                                                                unsigned int* __restrict__ queue) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int item_sh;
+  __shared__ uint32_t sweep0[N3_MAX_GROUPS], round0[N3_MAX_GROUPS], swaps[N3_MAX_GROUPS];
+  const int wave = threadIdx.x / kWave, lane = lane_id(), block_waves = blockDim.x / kWave;
+  const int count = t->count, total_items = t->total_items;
+  if (threadIdx.x < count) {
+    sweep0[threadIdx.x] = t->counters[threadIdx.x][0];
+    round0[threadIdx.x] = t->counters[threadIdx.x][1];
+  }
+  __syncthreads();
+  int staged = -1;
+  for (int s = 0; s < sweeps; ++s) {
+    unsigned int* next = queue + (s & 1);
+    for (;;) {
+      if (threadIdx.x == 0) item_sh = (int)atomicAdd(next, 1u);
+      __syncthreads();
+      const int item = item_sh;
+      __syncthreads();  // every thread has read the item before the next grab
+      if (item >= total_items) break;
+      int g = 0;
+      while (g + 1 < count && item >= t->walk[g + 1].block_begin) ++g;
+      const BatchGroup& G = t->walk[g];
+      const DirTableInfo info = G.info;
+      uint16_t* table = reinterpret_cast<uint16_t*>(smem);
+      uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+      if (g != staged) {
+        stage_table(G.table, table, info.entries);
+        if (wave < G.waves)
+          for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+        staged = g;
+        __syncthreads();
+      }
+      const int slot = (item - G.block_begin) * G.waves + wave;
+      if (wave < G.waves && slot < G.walk.replicas) {
+        const uint32_t sweep = sweep0[g] + (uint32_t)s;
+        switch (G.ch) {
+          case 1: walk_replica<1>(G.walk, table, hist, info, slot, sweep); break;
+          case 2: walk_replica<2>(G.walk, table, hist, info, slot, sweep); break;
+          case 3: walk_replica<3>(G.walk, table, hist, info, slot, sweep); break;
+          default: walk_replica<4>(G.walk, table, hist, info, slot, sweep); break;
+        }
+      }
+      __syncthreads();  // the item is done before the next one may restage the table
+    }
+    grid.sync();
+    if (threadIdx.x < N3_MAX_GROUPS) swaps[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) queue[(s + 1) & 1] = 0u;  // last used by sweep s - 1
+    __syncthreads();
+    const int pairs = t->exchange_begin[count], stride = (int)gridDim.x * block_waves;
+    for (int w = (int)blockIdx.x * block_waves + wave; w < pairs; w += stride) {
+      int g = 0;
+      while (g + 1 < count && w >= t->exchange_begin[g + 1]) ++g;
+      if (t->round_step[g] != 0 &&
+          exchange_pair(t->exchange[g], w - t->exchange_begin[g], round0[g] + (uint32_t)s * t->round_step[g]) &&
+          lane == 0)
+        atomicAdd(&swaps[g], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < count; ++i)
+        if (swaps[i]) atomicAdd(reinterpret_cast<unsigned long long*>(t->counters[i] + 2), (unsigned long long)swaps[i]);
+    grid.sync();
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < count) {
+      t->counters[threadIdx.x][0] = sweep0[threadIdx.x] + (uint32_t)sweeps;
+      t->counters[threadIdx.x][1] = round0[threadIdx.x] + (uint32_t)sweeps * t->round_step[threadIdx.x];
+    }
+    if (threadIdx.x == 0 && sweeps > 0) queue[(sweeps - 1) & 1] = 0u;
   }
 }
 
@@ -755,6 +846,8 @@ struct BatchPlan {
   unsigned exchange_blocks = 0;
 };
 
+// A group's blocks hold block_waves replicas (wavefronts), or as many as fit
+// in LDS beside its direction table when block_waves is 0.
 static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
   int lds_limit = 0, compute_units = 0;
   int st = device_limits(&lds_limit, &compute_units);
@@ -806,6 +899,7 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
   }
   plan->host.exchange_begin[count] = exchange_waves;
   plan->host.count = count;
+  plan->host.total_items = blocks;
   plan->walk_blocks = (unsigned)blocks;
   plan->walk_threads = max_waves * kWave;
   plan->walk_lds = lds;
@@ -935,6 +1029,90 @@ extern "C" int n3_temper_batched_run(const n3_group* groups, int group_count, in
   return N3_OK;
 }
 
+// Plans (device tables and work-queue counters) of persistent batched runs,
+// cached like the graphs, at most kMaxCachedGraphs of them.
+struct PersistentEntry {
+  std::vector<unsigned char> key;
+  BatchPlan plan;
+  unsigned int* queue;
+  unsigned grid;
+  hipStream_t stream;
+};
+static std::vector<PersistentEntry> g_persistent;
+
+extern "C" int n3_temper_batched_persistent(const n3_group* groups, int group_count, int sweeps, void* stream) {
+  if (!groups || group_count < 1 || group_count > N3_MAX_GROUPS)
+    return set_error(N3_EINVAL, "group_count must be in [1, %d]", N3_MAX_GROUPS);
+  if (sweeps < 0) return set_error(N3_EINVAL, "negative sweeps");
+  for (int i = 0; i < group_count; ++i) {
+    int st = check_chains(&groups[i].chains);
+    if (st) return st;
+    if (groups[i].chains.strengths)
+      return set_error(N3_EINVAL, "umbrella potentials are supported by n3_temper_advance and n3_temper_run only");
+    if (groups[i].steps < 1 || groups[i].bad_bias < 0 || groups[i].bad_bias > 100)
+      return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
+    if (groups[i].ladder > 1) {
+      st = check_exchange(&groups[i].chains, groups[i].ladder);
+      if (st) return st;
+    }
+  }
+  if (sweeps == 0) return N3_OK;
+  hipStream_t s = as_stream(stream);
+  int device = 0;
+  (void)hipGetDevice(&device);
+  std::vector<unsigned char> key(sizeof(n3_group) * group_count + 2 * sizeof(intptr_t));
+  std::memcpy(key.data(), groups, sizeof(n3_group) * group_count);
+  const intptr_t extra[2] = {(intptr_t)s, (intptr_t)device};
+  std::memcpy(key.data() + sizeof(n3_group) * group_count, extra, sizeof(extra));
+  std::lock_guard<std::mutex> lock(g_graph_mutex);
+  PersistentEntry* entry = nullptr;
+  for (auto& e : g_persistent)
+    if (e.key == key) entry = &e;
+  if (!entry) {
+    PersistentEntry e;
+    e.key = key;
+    e.stream = s;
+    e.queue = nullptr;
+    int st = batch_plan(groups, group_count, &e.plan);
+    if (st) return st;
+    int coop = 0, compute_units = 0, per_cu = 0;
+    st = hip_status(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device), "hipDeviceGetAttribute");
+    if (!st && !coop) st = set_error(N3_ENODEV, "device %d does not support cooperative launches", device);
+    if (!st)
+      st = hip_status(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device),
+                      "hipDeviceGetAttribute");
+    if (!st)
+      st = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, walk_persistent_kernel,
+                                                                   e.plan.walk_threads, e.plan.walk_lds),
+                      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (!st && per_cu < 1) st = set_error(N3_EINVAL, "the persistent walk does not fit on a compute unit");
+    if (!st) st = hip_status(hipMalloc(&e.queue, 2 * sizeof(unsigned int)), "hipMalloc(work queue)");
+    if (!st) st = hip_status(hipMemset(e.queue, 0, 2 * sizeof(unsigned int)), "hipMemset(work queue)");
+    if (st) {
+      (void)hipFree(e.plan.device);
+      if (e.queue) (void)hipFree(e.queue);
+      return st;
+    }
+    e.grid = (unsigned)(per_cu * compute_units);
+    if (g_persistent.size() >= kMaxCachedGraphs) {
+      PersistentEntry& old = g_persistent.front();
+      (void)hipStreamSynchronize(old.stream);
+      (void)hipFree(old.plan.device);
+      (void)hipFree(old.queue);
+      g_persistent.erase(g_persistent.begin());
+    }
+    g_persistent.push_back(std::move(e));
+    entry = &g_persistent.back();
+  }
+  BatchTables* tables = entry->plan.device;
+  unsigned int* queue = entry->queue;
+  void* args[] = {&tables, &sweeps, &queue};
+  return hip_status(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(walk_persistent_kernel),
+                                               dim3(entry->grid), dim3(entry->plan.walk_threads), args,
+                                               (unsigned)entry->plan.walk_lds, s),
+                    "hipLaunchCooperativeKernel(walk_persistent_kernel)");
+}
+
 extern "C" int n3_temper_advance(const n3_chains* chains, int steps, int bad_bias, uint64_t seed, void* stream) {
   int st = check_chains(chains);
   if (st) return st;
@@ -1007,12 +1185,19 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
 
 extern "C" void n3_temper_release(void) {
   std::lock_guard<std::mutex> lock(g_graph_mutex);
+  // nothing may still be running on the buffers and graphs freed below
+  (void)hipDeviceSynchronize();
   for (auto& e : g_graphs) {
     (void)hipGraphExecDestroy(e.exec);
     (void)hipGraphDestroy(e.graph);
     (void)hipFree(e.done);
   }
   g_graphs.clear();
+  for (auto& e : g_persistent) {
+    (void)hipFree(e.plan.device);
+    (void)hipFree(e.queue);
+  }
+  g_persistent.clear();
   for (auto& e : g_batches) {
     (void)hipGraphExecDestroy(e.exec);
     (void)hipGraphDestroy(e.graph);

@@ -112,3 +112,34 @@ def test_autotune_picks_a_fitting_shape(pkg, dev, tmp_path):
     assert result["block_waves"] in (1, 4, 16)
     table = pkg.autotune.tune({30: 128}, cache=tmp_path / "t.json", sweeps=1, candidates=(2, 8))
     assert table[30] in (2, 8) and (tmp_path / "t.json").exists()
+
+
+def test_persistent_launch_equals_per_sweep_launches(pkg, dev):
+    """All sweeps of config 4's six sizes in one cooperative launch give the
+    per-sweep launches' trajectories and counters bit for bit; a group
+    without exchanges and a badness-biased group ride along, and runs
+    continue from the device counters in either form."""
+    spec = [(n, 64, 16, 0) for n in CONFIG4] + [(20, 48, 0, 0), (30, 64, 16, 40)]
+    persistent, stepped = [], []
+    for i, (n, replicas, ladder, bias) in enumerate(spec):
+        for out in (persistent, stepped):
+            chains = pkg.seeded_chains(n, replicas, max(ladder, 16), seed=300 + i)
+            chains.block_waves = [4, 8, 16][i % 3] if n <= 75 else 4
+            out.append(pkg.BatchGroup(chains, ladder, 2 * n, bias))
+    for sweeps in (7, 3):
+        print(f"persistent vs per-sweep launches: {sweeps} sweeps", flush=True)
+        pkg.run_batched(persistent, sweeps, persistent=True)
+        pkg.run_batched(stepped, sweeps)
+        torch.cuda.synchronize()
+        for a, b in zip(persistent, stepped):
+            for name in FIELDS:
+                assert torch.equal(getattr(a.chains, name), getattr(b.chains, name)), (a.chains.n, name)
+    for g in persistent:
+        g.chains.validate()
+    # mixed: per-sweep launches continue a persistent run and vice versa
+    pkg.run_batched(persistent, 2)
+    pkg.run_batched(stepped, 2, persistent=True)
+    torch.cuda.synchronize()
+    for a, b in zip(persistent, stepped):
+        for name in FIELDS:
+            assert torch.equal(getattr(a.chains, name), getattr(b.chains, name)), (a.chains.n, name)

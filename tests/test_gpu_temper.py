@@ -268,3 +268,32 @@ def test_driver_writes_the_reference_outputs(pkg, dev, tmp_path):
         assert meta["symmetric_difference"] // 8 == int(f.stem[len("frontier-bin"):])
     saved = np.load(umb / "checkpoint.npz")
     assert np.allclose(saved["strengths"], 0.2) and saved["centers"].max() == np.float32(12)
+
+
+def test_long_line_states_match_replay(pkg, oracle, dev):
+    """States with a 38-point line: P is the identity except rows 20 and 21
+    (P[20] = 21, P[21] = 20), so the diagonal holds 38 points, and the
+    2-switch of slots 40 and 42 puts both moved points back on it
+    ((20, 20) and (21, 21)).  The moved-state histograms then count 38
+    points in one direction, beyond what a 4-bit counter holds, and the walk
+    must take its byte-counter path for those proposals; every replica
+    stays bit-exact against the replay (the pair is drawn about once in
+    3,200 proposals: about 6 times per sweep over 256 replicas of 80)."""
+    n, replicas, ladder, sweeps = 40, 256, 16, 6
+    p = np.arange(n)
+    p[20], p[21] = 21, 20
+    q = (np.arange(n) + 7) % n
+    assert np.all(p != q) and q[20] != 20 and q[21] != 21
+    row = np.column_stack((p, q)).reshape(-1).astype(np.int32)
+    cols = np.tile(row, (replicas, 1))
+    temps = np.tile(np.geomspace(0.05, 2.0, ladder), replicas // ladder).astype(np.float32)
+    chains = pkg.TemperingChains(cols, temps, 2024)
+    replay = oracle.TemperingReplay(cols, temps, 2024, k_move=chains.k_move,
+                                    k_exchange=chains._exchange_table(ladder)[2])
+    for sweep in range(sweeps):
+        print(f"long-line walk vs replay: sweep {sweep}", flush=True)
+        chains.advance(2 * n)
+        replay.advance(2 * n, threads=oracle.cpu_count())
+        _compare(chains, replay)
+        assert chains.exchange(ladder) == replay.exchange(ladder)
+    chains.validate()

@@ -86,6 +86,15 @@ int n3_score_status(void* stream);
 int n3_two_switch_scores(const int32_t* p, const int32_t* q, const int32_t* badness, int n, int u,
                          int current_score, int32_t* scores, void* stream);
 
+/* n3_two_switch_scores without any synchronisation: the call only enqueues
+ * the kernel, which ORs 1 into *status (a device int32 the caller zeroes
+ * beforehand) when a column lies outside [0, n).  A caller that stages its
+ * inputs, the zeroed status word and the scores in one buffer then needs
+ * one copy in, one copy out and one synchronisation per call.  Same kernel
+ * and scores as n3_two_switch_scores (gpu_delta.py:18-116). */
+int n3_two_switch_scores_status(const int32_t* p, const int32_t* q, const int32_t* badness, int n, int u,
+                                int current_score, int32_t* scores, int32_t* status, void* stream);
+
 /* accelerated.evaluate_two_switches (accelerated.py:122-195) on the device:
  * int64 scores[2n] with the sentinel 1<<60; badness is int64[2n]. */
 int n3_evaluate_two_switches(const int32_t* p, const int32_t* q, const int64_t* badness, int n, int u,

@@ -93,6 +93,7 @@ SIGNATURES = {
     "n3_score_population": (_i, [_vp, _vp, _i, _i64, _vp, _vp, _i, _vp]),
     "n3_score_status": (_i, [_vp]),
     "n3_two_switch_scores": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
+    "n3_two_switch_scores_status": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
     "n3_evaluate_two_switches": (_i, [_vp, _vp, _vp, _i, _i, _i64, _vp, _vp]),
     "n3_evaluate_transpositions": (_i, [_vp, _vp, _vp, _i, _i, _i, _i64, _vp, _vp]),
     "n3_delta_table": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp]),

@@ -402,6 +402,16 @@ extern "C" int n3_two_switch_scores(const int32_t* p, const int32_t* q, const in
   return run_switch(a, false, as_stream(stream));
 }
 
+extern "C" int n3_two_switch_scores_status(const int32_t* p, const int32_t* q, const int32_t* badness, int n, int u,
+                                           int current_score, int32_t* scores, int32_t* status, void* stream) {
+  int st = check_switch(n, p, q, badness, scores);
+  if (st) return st;
+  if (!status) return set_error(N3_EINVAL, "null status");
+  if (u < 0 || u >= 2 * n) return set_error(N3_EINVAL, "selected index %d outside [0, %d)", u, 2 * n);
+  SwitchArgs a{p, q, n, 1, u, -1, 2 * n, badness, nullptr, current_score, scores, 2 * n, status};
+  return dispatch_switch<int32_t, int32_t>(a, as_stream(stream));
+}
+
 extern "C" int n3_evaluate_two_switches(const int32_t* p, const int32_t* q, const int64_t* badness, int n, int u,
                                         int64_t current_score, int64_t* scores, void* stream) {
   int st = check_switch(n, p, q, badness, scores);

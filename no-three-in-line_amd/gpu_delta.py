@@ -28,16 +28,21 @@ def score_two_switches_gpu(p, q, selected_index: int, current_score: int, badnes
     n = len(p_host)
     dev = _lib.device()
     with _lib.STAGING_LOCK:
-        st = _lib.staging(dev, 6 * n)
+        # staged words: P, Q, badness, the status word (zeroed here), the
+        # scores: one copy in, one copy out (status and scores), one sync
+        st = _lib.staging(dev, 6 * n + 1)
         h = st.host_np
-        h[:n], h[n:2 * n], h[2 * n:4 * n] = p_host, q_host, bad_host
-        st.dev[:4 * n].copy_(st.host[:4 * n], non_blocking=True)
+        h[:n], h[n:2 * n], h[2 * n:4 * n], h[4 * n] = p_host, q_host, bad_host, 0
+        st.dev[:4 * n + 1].copy_(st.host[:4 * n + 1], non_blocking=True)
         base = st.dev.data_ptr()
-        _lib.check(_lib.lib().n3_two_switch_scores(base, base + 4 * n, base + 8 * n, n, int(selected_index),
-                                                   int(current_score), base + 16 * n, _lib.stream_handle(dev)))
-        st.host[4 * n:6 * n].copy_(st.dev[4 * n:6 * n], non_blocking=True)
+        _lib.check(_lib.lib().n3_two_switch_scores_status(base, base + 4 * n, base + 8 * n, n, int(selected_index),
+                                                          int(current_score), base + 16 * n + 4, base + 16 * n,
+                                                          _lib.stream_handle(dev)))
+        st.host[4 * n:6 * n + 1].copy_(st.dev[4 * n:6 * n + 1], non_blocking=True)
         torch.cuda.current_stream(dev).synchronize()
-        return h[4 * n:6 * n].copy()
+        if h[4 * n] & 1:
+            raise ValueError("a column lies outside [0, n)")
+        return h[4 * n + 1:6 * n + 1].copy()
 
 
 def evaluate_two_switches(p, q, selected_index: int, current_score: int, badness) -> np.ndarray:

@@ -110,6 +110,12 @@ def test_coincident_points_delta(pkg, oracle, dev):
 def test_delta_shape_errors(pkg, dev):
     with pytest.raises(ValueError):
         pkg.score_two_switches_gpu(np.arange(5), np.arange(5), 0, 0, np.zeros(9))
+    # a column outside [0, n) is reported through the staged status word, and
+    # the next call on the same staging buffer starts from a clear status
+    p, q = np.arange(5), np.roll(np.arange(5), 1)
+    with pytest.raises(ValueError):
+        pkg.score_two_switches_gpu(p, np.where(q == 4, 7, q), 0, 0, np.zeros(10))
+    assert pkg.score_two_switches_gpu(p, q, 0, 0, np.zeros(10)).shape == (10,)
     with pytest.raises(ValueError):
         pkg.evaluate_two_switches(np.arange(5), np.roll(np.arange(5), 1), 10, 0, np.zeros(10))
 

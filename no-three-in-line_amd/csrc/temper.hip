@@ -384,7 +384,16 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
   stage_table(table_g, table, info.entries);
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
-  const int slot = blockIdx.x * waves + wave;
+  int slot = blockIdx.x * waves + wave;
+  if (a.ladder >= waves && a.ladder % waves == 0) {
+    // block b holds every (ladder / waves)-th temperature of one ladder, so
+    // that hot replicas (more acceptances, each with four read-backs) are
+    // spread over the blocks and SIMDs instead of filling the last blocks
+    // of every ladder; the slot keys the random stream, so trajectories do
+    // not depend on the mapping
+    const int per = a.ladder / waves;
+    slot = (int)(blockIdx.x / per) * a.ladder + wave * per + (int)(blockIdx.x % per);
+  }
   if (slot < a.replicas) walk_replica<CH, kUmbrella>(a, table, hist, info, slot, a.counters[0]);
 }
 
@@ -725,9 +734,11 @@ WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed) {
 // The walk; with bump_sweep the sweep counter is advanced by a counter kernel
 // after it (a following fused exchange may advance it instead).
 static int enqueue_advance(const n3_chains* c, const WalkPlan& plan, int steps, int bad_bias, uint64_t seed,
-                           hipStream_t stream, bool bump_sweep = true) {
-  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream,
-                     walk_args(c, steps, bad_bias, seed), plan.table, plan.info);
+                           hipStream_t stream, bool bump_sweep = true, int ladder = 0) {
+  WalkArgs a = walk_args(c, steps, bad_bias, seed);
+  a.ladder = ladder;
+  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream, a, plan.table,
+                     plan.info);
   int st = hip_status(hipGetLastError(), "walk_kernel launch");
   if (st || !bump_sweep) return st;
   hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, 1u, 0u);
@@ -813,7 +824,7 @@ static int capture_graph(const n3_chains* c, const WalkPlan& plan, int sweeps, i
   st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (!st) {
     for (int i = 0; i < sweeps && !st; ++i) {
-      st = enqueue_advance(c, plan, steps, bad_bias, seed, cap, ladder <= 1);
+      st = enqueue_advance(c, plan, steps, bad_bias, seed, cap, ladder <= 1, ladder);
       if (!st && ladder > 1) st = enqueue_exchange(c, ladder, seed, cap, done, 1u);
     }
     hipGraph_t g = nullptr;
@@ -1177,7 +1188,7 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
     if (st) return st;
   }
   for (int i = 0; i < rest && !st; ++i) {
-    st = enqueue_advance(chains, plan, steps, bad_bias, seed, s);
+    st = enqueue_advance(chains, plan, steps, bad_bias, seed, s, true, ladder);
     if (!st && ladder > 1) st = enqueue_exchange(chains, ladder, seed, s);
   }
   return st;

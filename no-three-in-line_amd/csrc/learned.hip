@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(1024) walk_learned_kernel(WalkArgs a, LearnedA
   stage_table(table_g, table, info.entries);
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
-  const int slot = blockIdx.x * waves + wave;
+  const int slot = walk_slot((int)blockIdx.x, wave, waves, a.ladder);
   if (slot < a.replicas) walk_replica_learned<CH>(a, L, table, hist, info, slot);
 }
 
@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(1024) walk_point_kernel(WalkArgs a, PointArgs 
   stage_point(P, cst, w1);
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
-  const int slot = blockIdx.x * waves + wave;
+  const int slot = walk_slot((int)blockIdx.x, wave, waves, a.ladder);
   if (slot < a.replicas) walk_replica_point<CH>(a, P, table, hist, cst, w1, info, slot);
 }
 
@@ -775,9 +775,11 @@ static int learned_plan(const n3_chains* c, LearnedPlan* plan) {
 }
 
 static int enqueue_learned(const n3_chains* c, const LearnedPlan& plan, const LearnedArgs& L, int steps, uint64_t seed,
-                           hipStream_t stream) {
-  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream,
-                     walk_args(c, steps, 0, seed), L, plan.table, plan.info);
+                           hipStream_t stream, int ladder) {
+  WalkArgs a = walk_args(c, steps, 0, seed);
+  a.ladder = ladder;
+  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream, a, L, plan.table,
+                     plan.info);
   int st = hip_status(hipGetLastError(), "walk_learned_kernel launch");
   if (st) return st;
   hipLaunchKernelGGL(bump_sweep_kernel, dim3(1), dim3(64), 0, stream, c->counters);
@@ -825,9 +827,11 @@ static int point_plan(const n3_chains* c, PointPlan* plan) {
 }
 
 static int enqueue_point(const n3_chains* c, const PointPlan& plan, const PointArgs& P, int steps, uint64_t seed,
-                         hipStream_t stream) {
-  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream,
-                     walk_args(c, steps, 0, seed), P, plan.table, plan.info);
+                         hipStream_t stream, int ladder) {
+  WalkArgs a = walk_args(c, steps, 0, seed);
+  a.ladder = ladder;
+  hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream, a, P, plan.table,
+                     plan.info);
   int st = hip_status(hipGetLastError(), "walk_point_kernel launch");
   if (st) return st;
   hipLaunchKernelGGL(bump_sweep_kernel, dim3(1), dim3(64), 0, stream, c->counters);
@@ -948,7 +952,7 @@ extern "C" int n3_temper_run_learned(const n3_chains* chains, const uint32_t* we
   if (st) return st;
   const LearnedArgs L{weights, temp_index, max_energy, reinterpret_cast<const unsigned long long*>(thr64), k64};
   return run_graphs(chains, &L, sizeof L, sweeps, steps, ladder, seed, as_stream(stream), [&](hipStream_t s) {
-    int e = enqueue_learned(chains, plan, L, steps, seed, s);
+    int e = enqueue_learned(chains, plan, L, steps, seed, s, ladder);
     if (!e && ladder > 1) e = enqueue_exchange(chains, ladder, seed, s);
     return e;
   });
@@ -974,7 +978,7 @@ extern "C" int n3_temper_run_learned_point(const n3_chains* chains, const uint16
   if (st) return st;
   const PointArgs P{consts, params, expw, lt_q, reinterpret_cast<const unsigned long long*>(thr64), k64};
   return run_graphs(chains, &P, sizeof P, sweeps, steps, ladder, seed, as_stream(stream), [&](hipStream_t s) {
-    int e = enqueue_point(chains, plan, P, steps, seed, s);
+    int e = enqueue_point(chains, plan, P, steps, seed, s, ladder);
     if (!e && ladder > 1) e = enqueue_exchange(chains, ladder, seed, s);
     return e;
   });

@@ -384,16 +384,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
   stage_table(table_g, table, info.entries);
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
-  int slot = blockIdx.x * waves + wave;
-  if (a.ladder >= waves && a.ladder % waves == 0) {
-    // block b holds every (ladder / waves)-th temperature of one ladder, so
-    // that hot replicas (more acceptances, each with four read-backs) are
-    // spread over the blocks and SIMDs instead of filling the last blocks
-    // of every ladder; the slot keys the random stream, so trajectories do
-    // not depend on the mapping
-    const int per = a.ladder / waves;
-    slot = (int)(blockIdx.x / per) * a.ladder + wave * per + (int)(blockIdx.x % per);
-  }
+  const int slot = walk_slot((int)blockIdx.x, wave, waves, a.ladder);
   if (slot < a.replicas) walk_replica<CH, kUmbrella>(a, table, hist, info, slot, a.counters[0]);
 }
 
@@ -424,7 +415,7 @@ __global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __
     for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
   if (wave >= G.waves) return;
-  const int slot = ((int)blockIdx.x - G.block_begin) * G.waves + wave;
+  const int slot = walk_slot((int)blockIdx.x - G.block_begin, wave, G.waves, G.walk.ladder);
   if (slot >= G.walk.replicas) return;
   switch (G.ch) {
     case 1: walk_replica<1>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
@@ -614,6 +605,9 @@ __global__ void __launch_bounds__(1024) walk_persistent_kernel(BatchTables* __re
         staged = g;
         __syncthreads();
       }
+      // consecutive slots: the block waits for the slowest wave of each
+      // item, so an item of neighbouring temperatures (alike in cost) keeps
+      // the waits short; spreading an item over a ladder was 2% slower here
       const int slot = (item - G.block_begin) * G.waves + wave;
       if (wave < G.waves && slot < G.walk.replicas) {
         const uint32_t sweep = sweep0[g] + (uint32_t)s;
@@ -892,6 +886,7 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
     }
     if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", c->n);
     bg.walk = walk_args(c, gr.steps, gr.bad_bias, gr.seed);
+    bg.walk.ladder = gr.ladder > 1 ? gr.ladder : 0;
     bg.ch = (2 * c->n + kWave - 1) / kWave;
     bg.waves = waves;
     bg.block_begin = blocks;

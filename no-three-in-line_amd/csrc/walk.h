@@ -32,7 +32,7 @@ struct WalkArgs {
   const float* centers;           // umbrella potentials: null strengths = none
   const float* strengths;
   const double* inverse_temperatures;
-  int ladder;                     // > 0: spread each ladder's temperatures over its blocks (walk_kernel)
+  int ladder;                     // > 0: each block takes temperatures spread over one ladder (walk_slot)
 };
 
 }  // namespace no3

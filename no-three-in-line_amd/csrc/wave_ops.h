@@ -48,6 +48,21 @@ __device__ inline int slot_value(const int (&arr)[CH], int slot) {
   return r;
 }
 
+// Replica slot of wavefront `wave` of walk block `block` (`waves` per
+// block).  With a ladder that is a multiple of `waves`, block b holds every
+// (ladder / waves)-th temperature of one ladder, so that the hot replicas
+// (more acceptances, and each acceptance costs more than a rejection) are
+// spread over the blocks and SIMDs instead of filling the last blocks of
+// every ladder; otherwise slots are consecutive.  The slot keys the random
+// stream, so trajectories do not depend on the mapping.
+__device__ inline int walk_slot(int block, int wave, int waves, int ladder) {
+  if (ladder >= waves && ladder % waves == 0) {
+    const int per = ladder / waves;
+    return (block / per) * ladder + wave * per + block % per;
+  }
+  return block * waves + wave;
+}
+
 // Entry `index` (uniform, >= 0) of a small per-replica table `global` whose
 // first 128 entries the wavefront holds in registers, entry lane + 64 j in
 // register j (lo: j = 0, hi: j = 1); later entries are read from memory.

@@ -258,3 +258,24 @@ def test_pretraining_is_reproducible(pkg, dev):
     a, _ = pkg.learned.pretrain(40, 1024, 64, seed=3, sweeps=20, log_sweeps=2, steps=60)
     b, _ = pkg.learned.pretrain(40, 1024, 64, seed=3, sweeps=20, log_sweeps=2, steps=60)
     assert torch.equal(a.master, b.master) and torch.equal(a.params, b.params)
+
+
+def test_block_mapping_leaves_trajectories_unchanged(pkg, dev):
+    """With 2,048 replicas the learned walk runs 8 replicas per block, and a
+    run with exchanges takes each block's replicas from every eighth
+    temperature of a 64-temperature ladder.  Launching the walk without the
+    ladder (consecutive slots per block) and the exchanges separately gives
+    the same trajectories: the slot, not the block, keys the random stream."""
+    net = _net(pkg, dev, np.random.default_rng(3))
+    a = pkg.seeded_chains(40, 2048, 64, seed=9)
+    b = pkg.seeded_chains(40, 2048, 64, seed=9)
+    pa, pb = pkg.learned.prepare(a, net), pkg.learned.prepare(b, net)
+    pkg.learned.run_learned(a, pa, 3, 80, 64)
+    for _ in range(3):
+        pkg.learned.run_learned(b, pb, 1, 80)
+        b.exchange(64)
+    torch.cuda.synchronize()
+    for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted", "removed_points",
+                 "counters", "proposals"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+    assert b.exchanges_total() > 0

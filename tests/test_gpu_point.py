@@ -154,3 +154,22 @@ def test_trained_point_network_reaches_low_energy_in_fewer_sweeps(pkg, dev):
     assert s_learned is not None
     assert s_uniform is None or s_learned < s_uniform, (s_learned, s_uniform)
     learned.validate(range(0, 4096, 257))
+
+
+def test_block_mapping_leaves_trajectories_unchanged(pkg, dev):
+    """As for the table mode: 2,048 replicas run 8 per block, and the walk
+    with the ladder (each block's replicas spread over a ladder) equals the
+    walk without it plus separate exchanges."""
+    net = _network(pkg, dev, seed=4)
+    a = pkg.seeded_chains(40, 2048, 64, seed=9)
+    b = pkg.seeded_chains(40, 2048, 64, seed=9)
+    pa, pb = pkg.learned.prepare_point(a, net), pkg.learned.prepare_point(b, net)
+    pkg.learned.run_learned_point(a, pa, 3, 80, 64)
+    for _ in range(3):
+        pkg.learned.run_learned_point(b, pb, 1, 80)
+        b.exchange(64)
+    torch.cuda.synchronize()
+    for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted", "removed_points",
+                 "counters", "proposals"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+    assert b.exchanges_total() > 0

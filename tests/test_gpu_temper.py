@@ -125,6 +125,26 @@ def test_graph_run_matches_step_by_step(pkg, oracle, dev):
     assert torch.equal(a2.columns, b.columns) and torch.equal(a2.scores, b.scores)
 
 
+@pytest.mark.parametrize("n,replicas,ladder", [(75, 4096, 64), (40, 2048, 64)])
+def test_block_mapping_leaves_trajectories_unchanged(pkg, dev, n, replicas, ladder):
+    """A run with exchanges takes each block's replicas from temperatures
+    spread over one ladder (4,096 replicas: 16 per block, every fourth
+    temperature; 2,048: 8 per block, every eighth); advance() without the
+    ladder keeps consecutive slots per block.  Both give the same
+    trajectories: the slot, not the block, keys the random stream."""
+    a = pkg.seeded_chains(n, replicas, ladder, seed=9)
+    b = pkg.seeded_chains(n, replicas, ladder, seed=9)
+    a.run(3, 2 * n, ladder)
+    for _ in range(3):
+        b.advance(2 * n)
+        b.exchange(ladder)
+    torch.cuda.synchronize()
+    for name in ("columns", "badness", "scores", "best_columns", "best_scores", "accepted", "removed_points",
+                 "counters", "proposals"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+    assert b.exchanges_total() > 0
+
+
 def test_long_run_validates_and_improves(pkg, oracle, dev):
     n, replicas, ladder = 75, 256, 64
     cols = _states(oracle, n, replicas, seed=2)

@@ -77,8 +77,10 @@ __device__ inline double umbrella_energy(int delta, int dd, int removed, float s
 }
 
 // One replica slot's `a.steps` proposals; `table` and `hist` are this
-// wavefront's LDS direction table and histogram.
-template <int CH, bool kUmbrella = false>
+// wavefront's LDS direction table and histogram.  kPlain: the caller
+// guarantees no proposal log and no badness bias, and the instantiation
+// carries neither (fewer live scalar values in the loop).
+template <int CH, bool kUmbrella = false, bool kPlain = false>
 __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t* hist, const DirTableInfo& info,
                              int slot_in, uint32_t sweep) {
   const int lane = lane_id();
@@ -142,7 +144,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
   // one atomic (entry log_base + step); a reservation that does not fit in
   // the capacity logs nothing, and invalid proposals leave their entry as is
   int32_t* log_base = nullptr;
-  if (a.log) {
+  if (!kPlain && a.log) {
     unsigned long long first = 0;
     if (lane == 0) first = atomicAdd(a.log_count, (unsigned long long)a.steps);
     first = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(first >> 32), 0) << 32) |
@@ -164,14 +166,15 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
       asm volatile("" : "+s"(k0), "+s"(k1));
       philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagMove, k0, k1, move_r);
-      if (a.bad_bias > 0) philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagBias, k0, k1, bias_r);
+      if (!kPlain && a.bad_bias > 0)
+        philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagBias, k0, k1, bias_r);
     }
     const int draw = step - batch;
     uint32_t r[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) r[i] = (uint32_t)__builtin_amdgcn_readlane((int)move_r[i], draw);
     int u = (int)mulhi32(r[0], (uint32_t)m);
-    if (a.bad_bias > 0 && score > 0 && (int)mulhi32(r[3], 100u) < a.bad_bias) {
+    if (!kPlain && a.bad_bias > 0 && score > 0 && (int)mulhi32(r[3], 100u) < a.bad_bias) {
       // tempering.py:51-55: u drawn in proportion to badness
       const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readlane((int)bias_r[0], draw);
       const int target = (int)mulhi32(rb0, (uint32_t)(3 * score));
@@ -220,7 +223,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     };
     // the log records the exact delta of every valid proposal, and the
     // umbrella test has no integer bound
-    const bool exact = log_base != nullptr || kUmbrella;
+    const bool exact = (!kPlain && log_base != nullptr) || kUmbrella;
     if (exact) old_joint();
     // histograms seen from u' and v' in the moved state.  The rejection path
     // needs only bad'(u') and bad'(v') = sum_d C(cnt_d, 2), the sums of the
@@ -294,7 +297,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       accept = delta <= 0;
       if (!accept) accept = r[2] < register_table(thr_lo, thr_hi, thr, delta < a.k_move ? delta : a.k_move - 1);
     }
-    if (log_base && lane == 0) {
+    if (!kPlain && log_base && lane == 0) {
       // (badness u, badness v, energy, slot, delta, accepted, u | v << 16,
       // column u | column v << 16): training data for the proposal network
       int32_t* e = log_base + kLogFields * step;
@@ -373,7 +376,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
   }
 }
 
-template <int CH, bool kUmbrella = false>
+template <int CH, bool kUmbrella = false, bool kPlain = false>
 __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* __restrict__ table_g,
                                                    DirTableInfo info) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -385,7 +388,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
   const int slot = walk_slot((int)blockIdx.x, wave, waves, a.ladder);
-  if (slot < a.replicas) walk_replica<CH, kUmbrella>(a, table, hist, info, slot, a.counters[0]);
+  if (slot < a.replicas) walk_replica<CH, kUmbrella, kPlain>(a, table, hist, info, slot, a.counters[0]);
 }
 
 // Several populations of different sizes in one launch: block b belongs to
@@ -401,6 +404,8 @@ struct BatchGroup {
   int pad;
 };
 
+// kPlain: no group has a proposal log or a badness bias (see walk_replica).
+template <bool kPlain>
 __global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __restrict__ groups, int group_count) {
   extern __shared__ __align__(16) unsigned char smem[];
   int g = 0;
@@ -418,10 +423,10 @@ __global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __
   const int slot = walk_slot((int)blockIdx.x - G.block_begin, wave, G.waves, G.walk.ladder);
   if (slot >= G.walk.replicas) return;
   switch (G.ch) {
-    case 1: walk_replica<1>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    case 2: walk_replica<2>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    case 3: walk_replica<3>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    default: walk_replica<4>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 1: walk_replica<1, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 2: walk_replica<2, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 3: walk_replica<3, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    default: walk_replica<4, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
   }
 }
 
@@ -569,6 +574,7 @@ __global__ void __launch_bounds__(1024) exchange_batched_kernel(BatchTables* __r
 // each is cleared for its next use during the sweep before it, and both are
 // zero again when the launch ends).  Every trajectory is bit-identical to
 // the per-sweep launches'.
+template <bool kPlain>
 __global__ void __launch_bounds__(1024) walk_persistent_kernel(BatchTables* __restrict__ t, int sweeps,
                                                                 unsigned int* __restrict__ queue) {
   namespace cg = cooperative_groups;
@@ -612,10 +618,10 @@ __global__ void __launch_bounds__(1024) walk_persistent_kernel(BatchTables* __re
       if (wave < G.waves && slot < G.walk.replicas) {
         const uint32_t sweep = sweep0[g] + (uint32_t)s;
         switch (G.ch) {
-          case 1: walk_replica<1>(G.walk, table, hist, info, slot, sweep); break;
-          case 2: walk_replica<2>(G.walk, table, hist, info, slot, sweep); break;
-          case 3: walk_replica<3>(G.walk, table, hist, info, slot, sweep); break;
-          default: walk_replica<4>(G.walk, table, hist, info, slot, sweep); break;
+          case 1: walk_replica<1, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+          case 2: walk_replica<2, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+          case 3: walk_replica<3, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+          default: walk_replica<4, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
         }
       }
       __syncthreads();  // the item is done before the next one may restage the table
@@ -685,7 +691,8 @@ struct WalkPlan {
   void (*kernel)(WalkArgs, const uint16_t*, DirTableInfo);
 };
 
-static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan, bool umbrella = false) {
+static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan, bool umbrella = false,
+                     bool plain = false) {
   int st = device_dirtable(n, &plan->table, &plan->info);
   if (st) return st;
   int lds_limit = 0, compute_units = 0;
@@ -708,11 +715,12 @@ static int walk_plan(int n, int replicas, int block_waves, WalkPlan* plan, bool 
   plan->blocks = (unsigned)((replicas + waves - 1) / waves);
   plan->lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
   const int ch = (2 * n + kWave - 1) / kWave;
+  // the plain instantiation when there is no umbrella, log or badness bias
   switch (ch) {
-    case 1: plan->kernel = umbrella ? walk_kernel<1, true> : walk_kernel<1>; break;
-    case 2: plan->kernel = umbrella ? walk_kernel<2, true> : walk_kernel<2>; break;
-    case 3: plan->kernel = umbrella ? walk_kernel<3, true> : walk_kernel<3>; break;
-    default: plan->kernel = umbrella ? walk_kernel<4, true> : walk_kernel<4>; break;
+    case 1: plan->kernel = umbrella ? walk_kernel<1, true> : plain ? walk_kernel<1, false, true> : walk_kernel<1>; break;
+    case 2: plan->kernel = umbrella ? walk_kernel<2, true> : plain ? walk_kernel<2, false, true> : walk_kernel<2>; break;
+    case 3: plan->kernel = umbrella ? walk_kernel<3, true> : plain ? walk_kernel<3, false, true> : walk_kernel<3>; break;
+    default: plan->kernel = umbrella ? walk_kernel<4, true> : plain ? walk_kernel<4, false, true> : walk_kernel<4>; break;
   }
   return N3_OK;
 }
@@ -849,11 +857,13 @@ struct BatchPlan {
   int walk_threads = 0;
   size_t walk_lds = 0;
   unsigned exchange_blocks = 0;
+  bool plain = true;  // no group has a proposal log or a badness bias
 };
 
 // A group's blocks hold block_waves replicas (wavefronts), or as many as fit
 // in LDS beside its direction table when block_waves is 0.
 static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
+  plan->plain = true;
   int lds_limit = 0, compute_units = 0;
   int st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
@@ -886,6 +896,7 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
     }
     if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", c->n);
     bg.walk = walk_args(c, gr.steps, gr.bad_bias, gr.seed);
+    if (c->log || gr.bad_bias > 0) plan->plain = false;
     bg.walk.ladder = gr.ladder > 1 ? gr.ladder : 0;
     bg.ch = (2 * c->n + kWave - 1) / kWave;
     bg.waves = waves;
@@ -924,8 +935,9 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
 static int enqueue_batched_sweep(const BatchPlan& plan, hipStream_t stream) {
   // the group array is the first member of the device tables
   static_assert(offsetof(BatchTables, walk) == 0, "walk groups lead the batch tables");
-  hipLaunchKernelGGL(walk_batched_kernel, dim3(plan.walk_blocks), dim3(plan.walk_threads), plan.walk_lds, stream,
-                     reinterpret_cast<const BatchGroup*>(plan.device), plan.host.count);
+  hipLaunchKernelGGL(plan.plain ? walk_batched_kernel<true> : walk_batched_kernel<false>, dim3(plan.walk_blocks),
+                     dim3(plan.walk_threads), plan.walk_lds, stream, reinterpret_cast<const BatchGroup*>(plan.device),
+                     plan.host.count);
   int st = hip_status(hipGetLastError(), "walk_batched_kernel launch");
   if (st) return st;
   if (plan.exchange_blocks > 0) {
@@ -1088,7 +1100,8 @@ extern "C" int n3_temper_batched_persistent(const n3_group* groups, int group_co
       st = hip_status(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device),
                       "hipDeviceGetAttribute");
     if (!st)
-      st = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, walk_persistent_kernel,
+      st = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.plan.plain ? walk_persistent_kernel<true>
+                                                                                   : walk_persistent_kernel<false>,
                                                                    e.plan.walk_threads, e.plan.walk_lds),
                       "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (!st && per_cu < 1) st = set_error(N3_EINVAL, "the persistent walk does not fit on a compute unit");
@@ -1113,7 +1126,9 @@ extern "C" int n3_temper_batched_persistent(const n3_group* groups, int group_co
   BatchTables* tables = entry->plan.device;
   unsigned int* queue = entry->queue;
   void* args[] = {&tables, &sweeps, &queue};
-  return hip_status(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(walk_persistent_kernel),
+  const void* kernel = entry->plan.plain ? reinterpret_cast<const void*>(walk_persistent_kernel<true>)
+                                          : reinterpret_cast<const void*>(walk_persistent_kernel<false>);
+  return hip_status(hipLaunchCooperativeKernel(kernel,
                                                dim3(entry->grid), dim3(entry->plan.walk_threads), args,
                                                (unsigned)entry->plan.walk_lds, s),
                     "hipLaunchCooperativeKernel(walk_persistent_kernel)");
@@ -1125,7 +1140,8 @@ extern "C" int n3_temper_advance(const n3_chains* chains, int steps, int bad_bia
   if (steps < 1 || bad_bias < 0 || bad_bias > 100)
     return set_error(N3_EINVAL, "steps must be positive and bad_bias in [0,100]");
   WalkPlan plan;
-  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan, chains->strengths != nullptr);
+  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan, chains->strengths != nullptr,
+                 !chains->strengths && !chains->log && bad_bias == 0);
   if (st) return st;
   return enqueue_advance(chains, plan, steps, bad_bias, seed, as_stream(stream));
 }
@@ -1151,7 +1167,8 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
   }
   if (sweeps == 0) return N3_OK;
   WalkPlan plan;
-  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan, chains->strengths != nullptr);
+  st = walk_plan(chains->n, chains->replicas, chains->block_waves, &plan, chains->strengths != nullptr,
+                 !chains->strengths && !chains->log && bad_bias == 0);
   if (st) return st;
   hipStream_t s = as_stream(stream);
   const int per_graph = sweeps < 32 ? sweeps : 32;

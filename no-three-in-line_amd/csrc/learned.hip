@@ -158,7 +158,11 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
   for (int step = 0; step < a.steps; ++step) {
     if (step - batch >= kWave) {
       batch = step;
-      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagLearn, a.seed_lo, a.seed_hi, learn_r);
+      // opaque copies of the key: its round schedule is derived here, once
+      // per batch, instead of being kept live in scalar registers all loop
+      uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+      asm volatile("" : "+s"(k0), "+s"(k1));
+      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagLearn, k0, k1, learn_r);
     }
     uint32_t r[4];
 #pragma unroll
@@ -502,8 +506,11 @@ __device__ void walk_replica_point(const WalkArgs& a, const PointArgs& P, const 
   for (int step = 0; step < a.steps; ++step) {
     if (step - batch >= kWave) {
       batch = step;
-      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagLearn, a.seed_lo, a.seed_hi, learn_r);
-      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagHast, a.seed_lo, a.seed_hi, hast_r);
+      // opaque copies of the key (see walk_replica_learned)
+      uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+      asm volatile("" : "+s"(k0), "+s"(k1));
+      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagLearn, k0, k1, learn_r);
+      philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagHast, k0, k1, hast_r);
     }
     uint32_t r[4];
 #pragma unroll

@@ -165,17 +165,20 @@ def tempering_rate(pkg, torch, sweeps: int = 10000) -> dict:
     return out
 
 
-def batched_rate(pkg, torch, sweeps: int = 1000) -> dict:
+def batched_rate(pkg, torch, sweeps: int = 1000, persistent: bool = True) -> dict:
     """Config 4: mixed sizes n in {40, 52, 61, 75, 83, 100} with {8192, 4096,
     4096, 2048, 2048, 1024} replicas (ladders of 64, geometric 0.05..2.0), 2n
     proposals per sweep, replicas per workgroup from the autotuner (timing
     the persistent form of each size alone); all
     1,000 sweeps in ONE persistent cooperative launch (the leg's rate and
     roofline, HIP events around that launch), and the same sweeps as one
-    batched walk launch per sweep replayed from captured graphs."""
+    batched walk launch per sweep replayed from captured graphs.  Without
+    `persistent` (for profilers, which fail on programs that made a
+    cooperative launch), the shapes are tuned on and the leg reports the
+    per-sweep graphs only."""
     sizes = {40: 8192, 52: 4096, 61: 4096, 75: 2048, 83: 2048, 100: 1024}
     t0 = time.perf_counter()
-    shapes = pkg.autotune.tune(sizes, sweeps=8, candidates=(2, 4, 6, 8, 12, 16), refresh=True, persistent=True)
+    shapes = pkg.autotune.tune(sizes, sweeps=8, candidates=(2, 4, 6, 8, 12, 16), refresh=True, persistent=persistent)
     tune_seconds = time.perf_counter() - t0
     groups = []
     for i, (n, replicas) in enumerate(sizes.items()):
@@ -190,6 +193,11 @@ def batched_rate(pkg, torch, sweeps: int = 1000) -> dict:
     pkg.run_batched(groups, sweeps)
     torch.cuda.synchronize()
     graph_seconds = time.perf_counter() - t0
+    if not persistent:
+        return {"sizes": {str(n): r for n, r in sizes.items()}, "sweeps": sweeps, "launch": "per-sweep graphs",
+                "per_sweep_graphs": {"seconds": graph_seconds, "proposals_per_s": proposals / graph_seconds},
+                "proposals_per_s": proposals / graph_seconds,
+                "block_waves": {str(n): w for n, w in shapes.items()}, "autotune_seconds": tune_seconds}
     # the persistent launch (its plan is built by a short first run)
     pkg.run_batched(groups, 2, persistent=True)
     torch.cuda.synchronize()
@@ -558,6 +566,8 @@ def main() -> int:
     ap.add_argument("--ab-seconds", type=float, default=20.0, help="wall-clock budget of each A/B side")
     ap.add_argument("--near-seconds", type=float, default=4.0,
                     help="wall-clock budget of each side of the near-solution A/B")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="config 4 without the persistent cooperative launch (per-sweep graphs only; for profilers)")
     ap.add_argument("--method", type=int, default=0, help="0 auto (direction histograms), 1 determinant")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
@@ -628,7 +638,7 @@ def main() -> int:
         extra["tempering_n75_64x64"] = tempering_rate(pkg, torch, sweeps=args.temper_sweeps)
     if "batched" in legs:
         log("bench: batched uneven sizes")
-        extra["batched_uneven"] = batched_rate(pkg, torch)
+        extra["batched_uneven"] = batched_rate(pkg, torch, persistent=not args.no_persistent)
     if "learned" in legs:
         log("bench: learned vs uniform proposals")
         extra["learned_vs_uniform"] = learned_ab(pkg, args.ab_seconds)

@@ -168,8 +168,8 @@ def tempering_rate(pkg, torch, sweeps: int = 10000) -> dict:
 def batched_rate(pkg, torch, sweeps: int = 1000, persistent: bool = True) -> dict:
     """Config 4: mixed sizes n in {40, 52, 61, 75, 83, 100} with {8192, 4096,
     4096, 2048, 2048, 1024} replicas (ladders of 64, geometric 0.05..2.0), 2n
-    proposals per sweep, replicas per workgroup from the autotuner (timing
-    the persistent form of each size alone); all
+    proposals per sweep, replicas per workgroup from the autotuner (tuned on
+    the mixed run itself, in the form timed); all
     1,000 sweeps in ONE persistent cooperative launch (the leg's rate and
     roofline, HIP events around that launch), and the same sweeps as one
     batched walk launch per sweep replayed from captured graphs.  Without
@@ -178,7 +178,7 @@ def batched_rate(pkg, torch, sweeps: int = 1000, persistent: bool = True) -> dic
     per-sweep graphs only."""
     sizes = {40: 8192, 52: 4096, 61: 4096, 75: 2048, 83: 2048, 100: 1024}
     t0 = time.perf_counter()
-    shapes = pkg.autotune.tune(sizes, sweeps=8, candidates=(2, 4, 6, 8, 12, 16), refresh=True, persistent=persistent)
+    shapes = pkg.autotune.tune_mixed(sizes, sweeps=8, candidates=(4, 6, 8, 12, 16), refresh=True, persistent=persistent)
     tune_seconds = time.perf_counter() - t0
     groups = []
     for i, (n, replicas) in enumerate(sizes.items()):

@@ -114,6 +114,28 @@ def test_autotune_picks_a_fitting_shape(pkg, dev, tmp_path):
     assert table[30] in (2, 8) and (tmp_path / "t.json").exists()
 
 
+def test_mixed_autotune_tunes_the_batched_run(pkg, dev, tmp_path):
+    """The mixed tuner starts from the most waves that fit in LDS for every
+    n and returns, per n, a candidate that fits; its choice is cached per size
+    mix, and a batched run with it matches the replay-checked form."""
+    sizes = {40: 512, 100: 256}
+    res = pkg.autotune.tune_mixed_shapes(sizes, sweeps=1, candidates=(4, 8, 16))
+    shapes = {int(n): w for n, w in res["block_waves"].items()}
+    assert shapes[40] in (4, 8, 16) and shapes[100] in (4,) and res["trials"] >= 2
+    table = pkg.autotune.tune_mixed(sizes, cache=tmp_path / "m.json", sweeps=1, candidates=(4, 8, 16))
+    assert set(table) == {40, 100} and all(w in (4, 8, 16) for w in table.values())
+    assert (tmp_path / "m.json").exists()
+    groups = []
+    for i, (n, r) in enumerate(sizes.items()):
+        chains = pkg.seeded_chains(n, r, 64, seed=70 + i)
+        chains.block_waves = table[n]
+        groups.append(pkg.BatchGroup(chains, 64, 2 * n))
+    pkg.run_batched(groups, 3, persistent=True)
+    torch.cuda.synchronize()
+    for g in groups:
+        g.chains.validate(range(0, len(g.chains.temperatures_host), 37))
+
+
 def test_persistent_launch_equals_per_sweep_launches(pkg, dev):
     """All sweeps of config 4's six sizes in one cooperative launch give the
     per-sweep launches' trajectories and counters bit for bit; a group

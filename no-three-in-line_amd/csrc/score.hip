@@ -20,7 +20,9 @@
 // States with coincident points (P[r] == Q[r]) have no direction for the
 // pair; those states take the determinant path, which counts them exactly
 // like the reference (gpu.py:19-33).
+#include <algorithm>
 #include <mutex>
+#include <numeric>
 #include <type_traits>
 #include <vector>
 
@@ -340,8 +342,9 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
 //  kPacked16: the same table in a static 160 KiB array at byte 0 with wave
 //             w's histogram at byte (w + 4) * 8192, so that a counter word's
 //             address is the OR of the wave's base and the word's byte offset.
-//  kWide32:   4-byte entries (counter word's byte offset << 16 | bit shift of
-//             the counter in the word), laid out with dr fastest and a column
+//  kWide32:   4-byte entries (counter word's byte offset << 16 | lone-row
+//             bound << 8 | bit shift of the counter in the word), laid out
+//             with dr fastest and a column
 //             stride S = 16 (mod 32) words: the points of 16 consecutive rows
 //             that a half-wave looks up fall in distinct bank pairs, where
 //             the row-major ids fall in random banks.
@@ -510,13 +513,25 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
     // ctr[c]: the lane's counter word, computed once for the increment and
     // the read-back (the empty asm keeps the compiler from recomputing it on
     // both sides of the tail chunk's mask)
+    // Wide layout: byte 1 of an entry is the least row of an earlier slot b
+    // whose line through the anchor can hold another earlier point (255: no
+    // row can, 0: every row can).  A lane below it is alone on its line and
+    // skips the increment.  Its read-back finds 0 (no other lane shares its
+    // direction; the entries of 255 point at the zero vector's word), so the
+    // wide layout accumulates count - 1 clamped at 0: 0 for lone lanes and
+    // for lanes without an earlier slot alike.
+    constexpr bool kLone = (kLayout == kWide32);
+    auto alone = [&](int c, uint32_t e) -> bool {
+      if constexpr (kLone) return ((e >> 8) & 0xFFu) > (uint32_t)((lane + kWave * c) >> 1);
+      return false;
+    };
     auto increment = [&](int ca, int l, const uint32_t(&e)[CH], LdsWord* (&ctr)[CH]) {
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
         ctr[c] = word_at(counter(e[c]));
         asm volatile("" : "+v"(ctr[c]));
         const uint32_t one = 1u << (shift(e[c]) & 31u);
-        if (c < ca || before(l)) {
+        if ((c < ca || before(l)) & !alone(c, e[c])) {
           if constexpr (kLayout == kNibble16) {
             const uint32_t old = __hip_atomic_fetch_add(ctr[c], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             overflow |= __builtin_amdgcn_ubfe(old, shift(e[c]), 4) == 15u;
@@ -533,12 +548,14 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
 #pragma unroll
       for (int c = 0; c <= ca; ++c) *ctr[c] = 0u;
     };
-    // sum of the lanes' counts; each slot keeps its own running sum
+    // sum of the lanes' counts (wide layout: counts - 1, see above); each
+    // slot keeps its own running sum
     auto accumulate = [&](int ca, const uint32_t(&e)[CH], const uint32_t(&word)[CH]) {
       uint32_t part = 0;
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
-        const uint32_t cnt = __builtin_amdgcn_ubfe(word[c], shift(e[c]), counter_bits);
+        uint32_t cnt = __builtin_amdgcn_ubfe(word[c], shift(e[c]), counter_bits);
+        if constexpr (kLone) cnt = __builtin_elementwise_sub_sat(cnt, 1u);
         counts[c] += (int)cnt;
         part += cnt;
       }
@@ -572,8 +589,8 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
         // slots; its last-slot triples are (sum - a) / 2.
         const uint32_t total = wave_sum_dpp(p0 | (p1 << 16));
         const int a = kWave * ca + l;
-        const int first_own = (int)((total & 0xFFFFu) - (uint32_t)a) >> 1;
-        const int second_own = (int)((total >> 16) - (uint32_t)(a + 1)) >> 1;
+        const int first_own = (int)((total & 0xFFFFu) - (kLone ? 0u : (uint32_t)a)) >> 1;
+        const int second_own = (int)((total >> 16) - (kLone ? 0u : (uint32_t)(a + 1))) >> 1;
         // second first: when l + 1 is past the chunk (slot m or lane 64,
         // clamped to 63) the first write then restores lane l's value
         own[ca] = write_lane(second_own, min(l + 1, kWave - 1), own[ca]);
@@ -611,13 +628,13 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
       }
     }
     // slot k was an earlier slot of the anchors k+1 .. m-1, and each of them
-    // adds its count, which includes k itself
+    // adds its count, which includes k itself (wide layout: adds count - 1)
     int sum = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
       if (k < m) {
-        const int b = counts[c] - (m - 1 - k) + own[c];
+        const int b = counts[c] - (kLone ? 0 : (m - 1 - k)) + own[c];
         sum += b;
         if (badness) badness[s * m + k] = b;
       }
@@ -625,6 +642,26 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
     sum = wave_sum(sum);
     if (lane == 0) scores[s] = sum / 3;
   }
+}
+
+// The least row r_b for which the earlier slot b = a + (dr, dc) (dr <= 0) can
+// share its line through the anchor a with another earlier slot; a slot in a
+// lower row is alone on that line.  Earlier slots lie in rows <= row(a), so
+// the other points of the line that matter are the lattice points strictly
+// between a and b and those beyond b.
+//  - dr = 0: the other slot of a's own row, the only earlier slot in it: 255.
+//  - primitive step (p, q) = (dr, dc) / g with max(|p|, |q|) > (n - 1) / 2:
+//    no line of that direction meets three cells of the grid: 255.
+//  - g >= 2: lattice points lie between a and b: 0.
+//  - g = 1: the first point beyond b is b - (a - b), in row 2 r_b - r_a, so
+//    r_b >= r_a - r_b = -dr.
+// The zero offset (a lane without an earlier slot) gets 0.
+inline uint32_t lone_row_bound(int n, int dr, int dc) {
+  if (dr == 0) return dc == 0 ? 0u : 255u;
+  const int adr = dr < 0 ? -dr : dr, adc = dc < 0 ? -dc : dc;
+  const int g = std::gcd(adr, adc);
+  if (2 * std::max(adr / g, adc / g) > n - 1) return 255u;
+  return g >= 2 ? 0u : (uint32_t)adr;
 }
 
 // Column stride of the wide layout: at least n, and 16 (mod 32).
@@ -670,8 +707,11 @@ static int onesided_table(int n, int layout, const DirTableInfo& info, OneSidedT
     host.assign(((size_t)stride * w + 3) / 4 * 4, 0u);
     for (int dr = 0; dr <= span; ++dr)  // rows dr - span = -(n-1) .. 0
       for (int dc = 0; dc < w; ++dc) {
-        const uint32_t id = ids[(size_t)dr * w + dc];
-        host[(size_t)dr + (size_t)stride * dc] = ((id & ~3u) << 16) | ((id & 3u) << 3);
+        // a direction whose lines meet at most two cells counts in the zero
+        // vector's word, which no lane increments
+        const uint32_t need = lone_row_bound(n, dr - span, dc - span);
+        const uint32_t id = need == 255u ? (uint32_t)info.directions : ids[(size_t)dr * w + dc];
+        host[(size_t)dr + (size_t)stride * dc] = ((id & ~3u) << 16) | (need << 8) | ((id & 3u) << 3);
       }
     tab = OneSidedTable{nullptr, (int)(host.size() * 4), 1, stride, info.hist_words};
   } else {

@@ -531,7 +531,8 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
         ctr[c] = word_at(counter(e[c]));
         asm volatile("" : "+v"(ctr[c]));
         const uint32_t one = 1u << (shift(e[c]) & 31u);
-        if ((c < ca || before(l)) & !alone(c, e[c])) {
+        // one combined mask (a short-circuit && nests two exec masks)
+        if ((int)(c < ca || before(l)) & (int)!alone(c, e[c])) {
           if constexpr (kLayout == kNibble16) {
             const uint32_t old = __hip_atomic_fetch_add(ctr[c], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             overflow |= __builtin_amdgcn_ubfe(old, shift(e[c]), 4) == 15u;
@@ -667,6 +668,25 @@ inline uint32_t lone_row_bound(int n, int dr, int dc) {
 // Column stride of the wide layout: at least n, and 16 (mod 32).
 inline int wide_stride(int n) { return ((n + 15) / 32) * 32 + 16; }
 
+// Directions with a counter in the wide layout: the primitive (p, q), p > 0,
+// with max(p, |q|) <= (n - 1) / 2 (the lines that can meet three cells).
+static int wide_counters(int n) {
+  int count = 0;
+  for (int p = 1; 2 * p <= n - 1; ++p)
+    for (int q = -(n - 1) / 2; q <= (n - 1) / 2; ++q)
+      if (std::gcd(p, q < 0 ? -q : q) == 1) ++count;
+  return count;
+}
+
+// Words of one wave's wide-layout histogram: the counters, the shared word
+// after them, and at least 4n words (the determinant recount's scratch),
+// rounded to 16 words.
+static int wide_hist_words(int n) {
+  int words = ((wide_counters(n) + 3) & ~3) / 4 + 1;
+  if (words < 4 * n) words = 4 * n;
+  return (words + 15) / 16 * 16;
+}
+
 // Words of one wave's 4-bit-counter histogram: the D + 1 counters, at least
 // 4n words (the determinant recount's scratch), a multiple of 16 words.
 inline int nibble_hist_words(const DirTableInfo& info) {
@@ -703,17 +723,29 @@ static int onesided_table(int n, int layout, const DirTableInfo& info, OneSidedT
   std::vector<uint32_t> host;  // 32-bit words of the table, padded to 16 bytes
   OneSidedTable tab{};
   if (layout == kWide32) {
+    // Counters only for the directions whose lines can meet three cells of
+    // the grid, renumbered in id order.  The zero vector and every other
+    // direction share the word after them, which no lane increments.
+    const uint32_t zero_id = (uint32_t)info.directions;
+    std::vector<int> counter_id(zero_id + 1, -1);
+    for (int dr = 0; dr <= span; ++dr)
+      for (int dc = 0; dc < w; ++dc)
+        if (lone_row_bound(n, dr - span, dc - span) != 255u) counter_id[ids[(size_t)dr * w + dc]] = 0;
+    uint32_t counters = 0;
+    for (uint32_t id = 0; id < zero_id; ++id)
+      if (counter_id[id] == 0) counter_id[id] = (int)counters++;
+    const uint32_t zero_word_id = (counters + 3u) & ~3u;
+    if ((int)counters != wide_counters(n)) return set_error(N3_EINVAL, "wide table: counter count mismatch at n=%d", n);
     const int stride = wide_stride(n);
     host.assign(((size_t)stride * w + 3) / 4 * 4, 0u);
     for (int dr = 0; dr <= span; ++dr)  // rows dr - span = -(n-1) .. 0
       for (int dc = 0; dc < w; ++dc) {
-        // a direction whose lines meet at most two cells counts in the zero
-        // vector's word, which no lane increments
         const uint32_t need = lone_row_bound(n, dr - span, dc - span);
-        const uint32_t id = need == 255u ? (uint32_t)info.directions : ids[(size_t)dr * w + dc];
-        host[(size_t)dr + (size_t)stride * dc] = ((id & ~3u) << 16) | (need << 8) | ((id & 3u) << 3);
+        const uint32_t id = ids[(size_t)dr * w + dc];
+        const uint32_t cid = (need == 255u || id == zero_id) ? zero_word_id : (uint32_t)counter_id[id];
+        host[(size_t)dr + (size_t)stride * dc] = ((cid & ~3u) << 16) | (need << 8) | ((cid & 3u) << 3);
       }
-    tab = OneSidedTable{nullptr, (int)(host.size() * 4), 1, stride, info.hist_words};
+    tab = OneSidedTable{nullptr, (int)(host.size() * 4), 1, stride, wide_hist_words(n)};
   } else {
     const size_t entries = (size_t)n * w;
     host.assign((entries * 2 + 15) / 16 * 4, 0u);
@@ -750,7 +782,7 @@ static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t co
   const int wide_bytes = (wide_stride(n) * info.width * 4 + 15) / 16 * 16;
   OneSidedTable tab{table, (n * info.width * 2 + 15) / 16 * 16, info.width, 1, info.hist_words};
   int layout = kHalf16;
-  if (wide_bytes + kPackedWaves * hist_bytes <= lds_limit) {
+  if (wide_bytes + kPackedWaves * wide_hist_words(n) * 4 <= lds_limit) {
     layout = kWide32;
   } else if (tab.bytes <= kPackedTableBytes && hist_bytes <= kPackedHistBytes &&
              lds_limit >= kPackedTableBytes + kPackedWaves * kPackedHistBytes) {

@@ -23,7 +23,6 @@
 #include <algorithm>
 #include <mutex>
 #include <numeric>
-#include <type_traits>
 #include <vector>
 
 #include "runtime.h"
@@ -335,35 +334,18 @@ __global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __rest
 // ---------------------------------------------------------------------------
 
 // Anchor a sees only the slots b < a, all in rows <= row(a), so the path
-// stages only the direction-table rows dr = -(n-1) .. 0.
-// LDS layouts of the one-sided path.
-//  kHalf16:   the 2-byte direction ids of the half table (row-major, W per
-//             row), then one byte-counter histogram per wave.
-//  kPacked16: the same table in a static 160 KiB array at byte 0 with wave
-//             w's histogram at byte (w + 4) * 8192, so that a counter word's
-//             address is the OR of the wave's base and the word's byte offset.
-//  kWide32:   4-byte entries (counter word's byte offset << 16 | lone-row
-//             bound << 8 | bit shift of the counter in the word), laid out
-//             with dr fastest and a column
-//             stride S = 16 (mod 32) words: the points of 16 consecutive rows
-//             that a half-wave looks up fall in distinct bank pairs, where
-//             the row-major ids fall in random banks.
-//  kNibble16: 2-byte entries 4 id (row-major half table) and 4-bit counters,
-//             8 per word: half the histogram, so two 16-wave blocks share a CU.
-//             A counter that would pass 15 (16 earlier points on one line
-//             through the anchor) is detected from the returned old value
-//             and the state is recounted by the determinant path.
-enum OneSidedLayout { kHalf16 = 0, kPacked16 = 1, kWide32 = 2, kNibble16 = 3 };
-constexpr int kPackedTableBytes = 32 * 1024;
-constexpr int kPackedHistBytes = 8 * 1024;
-constexpr int kPackedWaves = 16;
-
+// stages only the direction-table rows dr = -(n-1) .. 0.  An entry is 4 bytes:
+//   counter word's byte offset << 16 | lone-row bound << 8 | counter shift,
+// laid out with dr fastest and a column stride S = 16 (mod 32) words, so the
+// points of the 16 consecutive rows that a half-wave looks up fall in
+// distinct bank pairs (row-major entries fall in random banks).  Byte
+// counters, four per word, exist only for the directions whose lines can
+// meet three cells of the grid (see lone_row_bound).
 struct OneSidedTable {
-  const void* data;   // entries of rows dr = -(n-1) .. 0
-  int bytes;          // staged bytes (multiple of 16)
-  int row_stride;     // entries between rows
-  int col_stride;     // entries between columns
-  int hist_words;     // 32-bit words of one wave's histogram
+  const uint32_t* data;  // entries of rows dr = -(n-1) .. 0
+  int bytes;             // staged bytes (multiple of 16)
+  int stride;            // entries between columns
+  int hist_words;        // 32-bit words of one wave's histogram
 };
 
 // After anchor a's histogram of the directions to the slots b < a is built,
@@ -374,42 +356,21 @@ struct OneSidedTable {
 // from later anchors plus its own last-slot count (accelerated.py:34-36),
 // with half the pairs of the two-sided histogram.  Anchors in the last,
 // partly filled chunk are the only ones that touch it.
-template <int CH, int kLayout>
+template <int CH>
 __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __restrict__ p,
                                                              const int32_t* __restrict__ q, int n, long long count,
                                                              int32_t* __restrict__ scores,
                                                              int32_t* __restrict__ badness, OneSidedTable tab,
                                                              DirTableInfo info, int* __restrict__ status) {
-  using Entry = std::conditional_t<kLayout == kWide32, uint32_t, uint16_t>;
   const int waves = blockDim.x / kWave;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = lane_id();
-  unsigned char* lds;
-  uint32_t hist_base;  // byte address of the wave's histogram
-  if constexpr (kLayout == kPacked16) {
-    __shared__ __align__(16) unsigned char packed[kPackedTableBytes + kPackedWaves * kPackedHistBytes];
-    lds = packed;
-    hist_base = (uint32_t)(wave + kPackedTableBytes / kPackedHistBytes) * kPackedHistBytes;
-  } else {
-    extern __shared__ __align__(16) unsigned char smem[];
-    lds = smem;
-    hist_base = (uint32_t)(tab.bytes + wave * tab.hist_words * 4);
-  }
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t hist_base = (uint32_t)(tab.bytes + wave * tab.hist_words * 4);  // byte address
   uint32_t* hist = reinterpret_cast<uint32_t*>(lds + hist_base);
-  // byte offset (from lds) of an entry's counter word, and the counter's
-  // bit position in it
-  auto counter = [&](uint32_t e) -> uint32_t {
-    if constexpr (kLayout == kWide32) return hist_base + (e >> 16);
-    if constexpr (kLayout == kNibble16) return hist_base + ((e >> 3) & ~3u);
-    if constexpr (kLayout == kPacked16) return hist_base | (e & ~3u);
-    return hist_base + (e & ~3u);
-  };
   using LdsWord = __attribute__((address_space(3))) uint32_t;
-  auto word_at = [&](uint32_t offset) { return (LdsWord*)reinterpret_cast<uint32_t*>(lds + offset); };
-  // the low five bits are the shift; v_lshlrev and v_bfe read only those
-  auto shift = [](uint32_t e) -> uint32_t {
-    if constexpr (kLayout == kWide32 || kLayout == kNibble16) return e;
-    return e << 3;
-  };
+  // an entry's counter word; the low five bits of the entry are the
+  // counter's shift, and v_lshlrev and v_bfe read only those
+  auto counter = [&](uint32_t e) { return (LdsWord*)reinterpret_cast<uint32_t*>(lds + hist_base + (e >> 16)); };
   {
     const uint2* src = reinterpret_cast<const uint2*>(tab.data);
     uint2* dst = reinterpret_cast<uint2*>(lds);
@@ -417,14 +378,12 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
   }
   for (int i = lane; i < tab.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
-  constexpr int counter_bits = (kLayout == kNibble16) ? 4 : 8;
 
   const int m = 2 * n;
   const int width = info.width;
-  constexpr int unit = (int)sizeof(Entry);
   const int span = n - 1;
   // byte offset of the entry of (dr, dc) = key_b - key_a + zero
-  const int zero = unit * (span * tab.row_stride + span * tab.col_stride);
+  const int zero = 4 * (span + span * tab.stride);
   for (long long s = (long long)blockIdx.x * waves + wave; s < count; s += (long long)gridDim.x * waves) {
     int key[CH];
     int outside = 0, coincide = 0;
@@ -477,17 +436,19 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
       continue;
     }
 
-    // at[c]: byte offset of the lane's point in the staged table's layout
-    int at[CH], counts[CH], own[CH];
+    // at[c]: byte offset of the lane's point in the staged table's layout;
+    // row[c]: its row
+    int at[CH], row[CH], counts[CH], own[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
-      at[c] = unit * ((k >> 1) * tab.row_stride + (key[c] - (k >> 1) * width) * tab.col_stride);
+      row[c] = k >> 1;
+      at[c] = 4 * (row[c] + (key[c] - row[c] * width) * tab.stride);
       counts[c] = own[c] = 0;
     }
     // Lanes without an earlier slot look up the zero vector's entry (one
     // broadcast address), read back and clear its word and skip only the
-    // increment: the zero vector's byte stays 0, so their read-back adds 0
+    // increment: no lane increments that word, so their read-back adds 0
     // and no LDS step waits behind a branch.
     const unsigned char* idle_entry = lds + zero;
     // One anchor (slot 64 ca + l) is: lookup of the earlier slots'
@@ -504,42 +465,29 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
         const unsigned char* entry = (c < ca || before(l)) ? from + at[c] : idle_entry;
-        e[c] = *reinterpret_cast<const Entry*>(entry);
+        e[c] = *reinterpret_cast<const uint32_t*>(entry);
       }
     };
-    // 4-bit counters: a lane whose increment finds 15 would carry into the
-    // next counter; it flags the state for the determinant recount
-    bool overflow = false;
+    // Byte 1 of an entry is the least row of an earlier slot b whose line
+    // through the anchor can hold another earlier point (255: no row can,
+    // 0: every row can).  A lane below it is alone on its line and skips the
+    // increment.  Its read-back finds 0 (no other lane shares its direction;
+    // the entries of 255 point at the word that no lane increments), so the
+    // lanes accumulate count - 1 clamped at 0: 0 for lone lanes and for
+    // lanes without an earlier slot alike.
+    auto alone = [&](int c, uint32_t e) -> bool { return ((e >> 8) & 0xFFu) > (uint32_t)row[c]; };
     // ctr[c]: the lane's counter word, computed once for the increment and
     // the read-back (the empty asm keeps the compiler from recomputing it on
-    // both sides of the tail chunk's mask)
-    // Wide layout: byte 1 of an entry is the least row of an earlier slot b
-    // whose line through the anchor can hold another earlier point (255: no
-    // row can, 0: every row can).  A lane below it is alone on its line and
-    // skips the increment.  Its read-back finds 0 (no other lane shares its
-    // direction; the entries of 255 point at the zero vector's word), so the
-    // wide layout accumulates count - 1 clamped at 0: 0 for lone lanes and
-    // for lanes without an earlier slot alike.
-    constexpr bool kLone = (kLayout == kWide32);
-    auto alone = [&](int c, uint32_t e) -> bool {
-      if constexpr (kLone) return ((e >> 8) & 0xFFu) > (uint32_t)((lane + kWave * c) >> 1);
-      return false;
-    };
+    // both sides of the increment's mask)
     auto increment = [&](int ca, int l, const uint32_t(&e)[CH], LdsWord* (&ctr)[CH]) {
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
-        ctr[c] = word_at(counter(e[c]));
+        ctr[c] = counter(e[c]);
         asm volatile("" : "+v"(ctr[c]));
-        const uint32_t one = 1u << (shift(e[c]) & 31u);
+        const uint32_t one = 1u << (e[c] & 31u);
         // one combined mask (a short-circuit && nests two exec masks)
-        if ((int)(c < ca || before(l)) & (int)!alone(c, e[c])) {
-          if constexpr (kLayout == kNibble16) {
-            const uint32_t old = __hip_atomic_fetch_add(ctr[c], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            overflow |= __builtin_amdgcn_ubfe(old, shift(e[c]), 4) == 15u;
-          } else {
-            __hip_atomic_fetch_add(ctr[c], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
-        }
+        if ((int)(c < ca || before(l)) & (int)!alone(c, e[c]))
+          __hip_atomic_fetch_add(ctr[c], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       wave_sync();
     };
@@ -549,93 +497,55 @@ __global__ void __launch_bounds__(1024) score_onesided_kernel(const int32_t* __r
 #pragma unroll
       for (int c = 0; c <= ca; ++c) *ctr[c] = 0u;
     };
-    // sum of the lanes' counts (wide layout: counts - 1, see above); each
-    // slot keeps its own running sum
+    // sum of the lanes' counts - 1; each slot keeps its own running sum
     auto accumulate = [&](int ca, const uint32_t(&e)[CH], const uint32_t(&word)[CH]) {
       uint32_t part = 0;
 #pragma unroll
       for (int c = 0; c <= ca; ++c) {
-        uint32_t cnt = __builtin_amdgcn_ubfe(word[c], shift(e[c]), counter_bits);
-        if constexpr (kLone) cnt = __builtin_elementwise_sub_sat(cnt, 1u);
+        const uint32_t cnt = __builtin_elementwise_sub_sat(__builtin_amdgcn_ubfe(word[c], e[c], 8), 1u);
         counts[c] += (int)cnt;
         part += cnt;
       }
       return part;
     };
+    // Anchors in pairs, from lane 0 of each chunk: the chunks hold 64 slots
+    // or, the last one, m - 64 (CH - 1), so every count is even.  Anchor 0
+    // has no earlier slot and adds nothing.  The last pair's lookahead reads
+    // the table for its own last lane again.
 #pragma unroll
     for (int ca = 0; ca < CH; ++ca) {
-      // slot 0 has no earlier slots
-      const int begin = (ca == 0) ? 1 : 0;
       const int limit = min(kWave, m - kWave * ca);
       uint32_t e0[CH], e1[CH], word0[CH], word1[CH];
       LdsWord* ctr[CH];
-      if (begin < limit) lookup(ca, begin, e0);
+      lookup(ca, 0, e0);
       wave_sync();
-      for (int l = begin; l < limit; l += 2) {
+      for (int l = 0; l < limit; l += 2) {
         increment(ca, l, e0, ctr);
         collect(ca, ctr, word0);
-        if (l + 1 < limit) lookup(ca, l + 1, e1);
+        lookup(ca, l + 1, e1);
         wave_sync();
         const uint32_t p0 = accumulate(ca, e0, word0);
-        uint32_t p1 = 0;
-        if (l + 1 < limit) {
-          increment(ca, l + 1, e1, ctr);
-          collect(ca, ctr, word1);
-          if (l + 2 < limit) lookup(ca, l + 2, e0);
-          wave_sync();
-          p1 = accumulate(ca, e1, word1);
-        }
-        // two anchors per reduction: an anchor's sum of counts is at most
-        // (m - 1)^2 < 2^16, so they share a word.  Anchor a has a earlier
-        // slots; its last-slot triples are (sum - a) / 2.
+        increment(ca, l + 1, e1, ctr);
+        collect(ca, ctr, word1);
+        lookup(ca, min(l + 2, limit - 1), e0);
+        wave_sync();
+        const uint32_t p1 = accumulate(ca, e1, word1);
+        // two anchors per reduction: an anchor's sum of counts - 1 is at
+        // most (m - 1)(m - 2) < 2^16, so they share a word; its last-slot
+        // triples are half of it
         const uint32_t total = wave_sum_dpp(p0 | (p1 << 16));
-        const int a = kWave * ca + l;
-        const int first_own = (int)((total & 0xFFFFu) - (kLone ? 0u : (uint32_t)a)) >> 1;
-        const int second_own = (int)((total >> 16) - (kLone ? 0u : (uint32_t)(a + 1))) >> 1;
-        // second first: when l + 1 is past the chunk (slot m or lane 64,
-        // clamped to 63) the first write then restores lane l's value
-        own[ca] = write_lane(second_own, min(l + 1, kWave - 1), own[ca]);
-        own[ca] = write_lane(first_own, l, own[ca]);
-      }
-    }
-    if constexpr (kLayout == kNibble16) {
-      if (__any(overflow)) {
-        // a line through some anchor holds 16 or more earlier points: the
-        // exact determinant count, with the (cleared) histogram as scratch
-        int32_t* cols = reinterpret_cast<int32_t*>(hist);
-        int32_t* credit = cols + m;
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          const int k = lane + kWave * c;
-          if (k < m) cols[k] = key[c] - (k >> 1) * width;
-        }
-        wave_sync();
-        int bad[CH];
-        determinant_state<CH>(cols, credit, m, bad);
-        int sum = 0;
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          const int k = lane + kWave * c;
-          if (k < m) {
-            sum += bad[c];
-            if (badness) badness[s * m + k] = bad[c];
-          }
-        }
-        sum = wave_sum(sum);
-        if (lane == 0) scores[s] = sum / 3;
-        for (int i = lane; i < 2 * m; i += kWave) hist[i] = 0;
-        wave_sync();
-        continue;
+        own[ca] = write_lane((int)(total >> 17), l + 1, own[ca]);
+        own[ca] = write_lane((int)((total & 0xFFFFu) >> 1), l, own[ca]);
       }
     }
     // slot k was an earlier slot of the anchors k+1 .. m-1, and each of them
-    // adds its count, which includes k itself (wide layout: adds count - 1)
+    // adds its count - 1 (0 where k is alone on their line)
     int sum = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
       if (k < m) {
-        const int b = counts[c] - (kLone ? 0 : (m - 1 - k)) + own[c];
+        const int b = counts[c] + own[c];
         sum += b;
         if (badness) badness[s * m + k] = b;
       }
@@ -665,12 +575,12 @@ inline uint32_t lone_row_bound(int n, int dr, int dc) {
   return g >= 2 ? 0u : (uint32_t)adr;
 }
 
-// Column stride of the wide layout: at least n, and 16 (mod 32).
-inline int wide_stride(int n) { return ((n + 15) / 32) * 32 + 16; }
+// Column stride of the table: at least n, and 16 (mod 32).
+inline int onesided_stride(int n) { return ((n + 15) / 32) * 32 + 16; }
 
-// Directions with a counter in the wide layout: the primitive (p, q), p > 0,
-// with max(p, |q|) <= (n - 1) / 2 (the lines that can meet three cells).
-static int wide_counters(int n) {
+// Directions with a counter: the primitive (p, q), p > 0, with
+// max(p, |q|) <= (n - 1) / 2 (the lines that can meet three cells).
+static int onesided_counters(int n) {
   int count = 0;
   for (int p = 1; 2 * p <= n - 1; ++p)
     for (int q = -(n - 1) / 2; q <= (n - 1) / 2; ++q)
@@ -678,29 +588,19 @@ static int wide_counters(int n) {
   return count;
 }
 
-// Words of one wave's wide-layout histogram: the counters, the shared word
-// after them, and at least 4n words (the determinant recount's scratch),
-// rounded to 16 words.
-static int wide_hist_words(int n) {
-  int words = ((wide_counters(n) + 3) & ~3) / 4 + 1;
+// Words of one wave's histogram: the counters, the shared word after them,
+// and at least 4n words (the determinant recount's scratch), rounded to 16.
+static int onesided_hist_words(int n) {
+  int words = ((onesided_counters(n) + 3) & ~3) / 4 + 1;
   if (words < 4 * n) words = 4 * n;
   return (words + 15) / 16 * 16;
 }
 
-// Words of one wave's 4-bit-counter histogram: the D + 1 counters, at least
-// 4n words (the determinant recount's scratch), a multiple of 16 words.
-inline int nibble_hist_words(const DirTableInfo& info) {
-  int words = (info.directions + 1 + 7) / 8;
-  if (words < 4 * info.n) words = 4 * info.n;
-  return (words + 15) / 16 * 16;
-}
-
-// Resident one-sided tables per (device, n, layout), built from the direction
-// ids on first use: kWide32 (4-byte entries, dr fastest, column stride S)
-// and kNibble16 (2-byte entries 4 id, row-major).
-static int onesided_table(int n, int layout, const DirTableInfo& info, OneSidedTable* out) {
+// Resident one-sided tables per (device, n), built from the direction ids on
+// first use.
+static int onesided_table(int n, const DirTableInfo& info, OneSidedTable* out) {
   struct Resident {
-    int device, n, layout;
+    int device, n;
     OneSidedTable tab;
   };
   static std::mutex mutex;
@@ -710,7 +610,7 @@ static int onesided_table(int n, int layout, const DirTableInfo& info, OneSidedT
   if (st) return st;
   std::lock_guard<std::mutex> lock(mutex);
   for (const auto& t : tables) {
-    if (t.device == device && t.n == n && t.layout == layout) {
+    if (t.device == device && t.n == n) {
       *out = t.tab;
       return N3_OK;
     }
@@ -720,39 +620,30 @@ static int onesided_table(int n, int layout, const DirTableInfo& info, OneSidedT
   st = build_dirtable(n, ids, built);
   if (st) return st;
   const int w = info.width, span = n - 1;
-  std::vector<uint32_t> host;  // 32-bit words of the table, padded to 16 bytes
-  OneSidedTable tab{};
-  if (layout == kWide32) {
-    // Counters only for the directions whose lines can meet three cells of
-    // the grid, renumbered in id order.  The zero vector and every other
-    // direction share the word after them, which no lane increments.
-    const uint32_t zero_id = (uint32_t)info.directions;
-    std::vector<int> counter_id(zero_id + 1, -1);
-    for (int dr = 0; dr <= span; ++dr)
-      for (int dc = 0; dc < w; ++dc)
-        if (lone_row_bound(n, dr - span, dc - span) != 255u) counter_id[ids[(size_t)dr * w + dc]] = 0;
-    uint32_t counters = 0;
-    for (uint32_t id = 0; id < zero_id; ++id)
-      if (counter_id[id] == 0) counter_id[id] = (int)counters++;
-    const uint32_t zero_word_id = (counters + 3u) & ~3u;
-    if ((int)counters != wide_counters(n)) return set_error(N3_EINVAL, "wide table: counter count mismatch at n=%d", n);
-    const int stride = wide_stride(n);
-    host.assign(((size_t)stride * w + 3) / 4 * 4, 0u);
-    for (int dr = 0; dr <= span; ++dr)  // rows dr - span = -(n-1) .. 0
-      for (int dc = 0; dc < w; ++dc) {
-        const uint32_t need = lone_row_bound(n, dr - span, dc - span);
-        const uint32_t id = ids[(size_t)dr * w + dc];
-        const uint32_t cid = (need == 255u || id == zero_id) ? zero_word_id : (uint32_t)counter_id[id];
-        host[(size_t)dr + (size_t)stride * dc] = ((cid & ~3u) << 16) | (need << 8) | ((cid & 3u) << 3);
-      }
-    tab = OneSidedTable{nullptr, (int)(host.size() * 4), 1, stride, wide_hist_words(n)};
-  } else {
-    const size_t entries = (size_t)n * w;
-    host.assign((entries * 2 + 15) / 16 * 4, 0u);
-    uint16_t* h16 = reinterpret_cast<uint16_t*>(host.data());
-    for (size_t i = 0; i < entries; ++i) h16[i] = (uint16_t)(4u * ids[i]);
-    tab = OneSidedTable{nullptr, (int)(host.size() * 4), w, 1, nibble_hist_words(info)};
-  }
+  // Counters only for the directions whose lines can meet three cells of
+  // the grid, renumbered in id order.  The zero vector and every other
+  // direction share the word after them, which no lane increments.
+  const uint32_t zero_id = (uint32_t)info.directions;
+  std::vector<int> counter_id(zero_id + 1, -1);
+  for (int dr = 0; dr <= span; ++dr)
+    for (int dc = 0; dc < w; ++dc)
+      if (lone_row_bound(n, dr - span, dc - span) != 255u) counter_id[ids[(size_t)dr * w + dc]] = 0;
+  uint32_t counters = 0;
+  for (uint32_t id = 0; id < zero_id; ++id)
+    if (counter_id[id] == 0) counter_id[id] = (int)counters++;
+  if ((int)counters != onesided_counters(n))
+    return set_error(N3_EINVAL, "one-sided table: %u counters at n=%d, expected %d", counters, n, onesided_counters(n));
+  const uint32_t shared_id = (counters + 3u) & ~3u;
+  const int stride = onesided_stride(n);
+  std::vector<uint32_t> host(((size_t)stride * w + 3) / 4 * 4, 0u);  // padded to 16 bytes
+  for (int dr = 0; dr <= span; ++dr)  // rows dr - span = -(n-1) .. 0
+    for (int dc = 0; dc < w; ++dc) {
+      const uint32_t need = lone_row_bound(n, dr - span, dc - span);
+      const uint32_t id = ids[(size_t)dr * w + dc];
+      const uint32_t cid = (need == 255u || id == zero_id) ? shared_id : (uint32_t)counter_id[id];
+      host[(size_t)dr + (size_t)stride * dc] = ((cid & ~3u) << 16) | (need << 8) | ((cid & 3u) << 3);
+    }
+  OneSidedTable tab{nullptr, (int)(host.size() * 4), stride, onesided_hist_words(n)};
   void* data = nullptr;
   st = hip_status(hipMalloc(&data, host.size() * 4), "hipMalloc(one-sided table)");
   if (st) return st;
@@ -761,66 +652,36 @@ static int onesided_table(int n, int layout, const DirTableInfo& info, OneSidedT
     (void)hipFree(data);
     return st;
   }
-  tab.data = data;
-  tables.push_back(Resident{device, n, layout, tab});
+  tab.data = static_cast<const uint32_t*>(data);
+  tables.push_back(Resident{device, n, tab});
   *out = tab;
   return N3_OK;
 }
 
-// Layout choice: the wide table where it leaves room for 16 histograms
-// (n <= 75), else the packed layout where it fits (n <= 80), else kHalf16
-// where 16 byte-counter histograms fit (n <= 83), else kNibble16, whose
-// histograms are half the size.  Two blocks share a CU when their LDS fits
-// (the wide layout at the smaller sizes).
+// Up to 16 waves per block; two blocks share a CU where their LDS fits
+// (n = 100 stages 87 KB of table and 3 KB per wave; n = 75 46.6 KB and
+// 1.75 KB per wave).
 template <int CH>
 static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
-                           const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
+                           const DirTableInfo& info, int* status, hipStream_t stream) {
   int lds_limit = 0, compute_units = 0;
   int st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
-  const int hist_bytes = info.hist_words * 4;
-  const int wide_bytes = (wide_stride(n) * info.width * 4 + 15) / 16 * 16;
-  OneSidedTable tab{table, (n * info.width * 2 + 15) / 16 * 16, info.width, 1, info.hist_words};
-  int layout = kHalf16;
-  if (wide_bytes + kPackedWaves * wide_hist_words(n) * 4 <= lds_limit) {
-    layout = kWide32;
-  } else if (tab.bytes <= kPackedTableBytes && hist_bytes <= kPackedHistBytes &&
-             lds_limit >= kPackedTableBytes + kPackedWaves * kPackedHistBytes) {
-    layout = kPacked16;
-  } else if (tab.bytes + 16 * hist_bytes > lds_limit) {
-    layout = kNibble16;
-  }
-  if (layout == kWide32 || layout == kNibble16) {
-    st = onesided_table(n, layout, info, &tab);
-    if (st) return st;
-  }
-  int waves = kPackedWaves, per_cu = 1;
-  size_t lds = 0;
-  if (layout != kPacked16) {
-    const int bytes = tab.hist_words * 4;
-    waves = (lds_limit - tab.bytes) / bytes;
-    if (waves > 16) waves = 16;
-    if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
-    lds = (size_t)tab.bytes + (size_t)waves * bytes;
-    if (2 * lds <= (size_t)lds_limit) per_cu = 2;  // two blocks share a CU
-  }
+  OneSidedTable tab{};
+  st = onesided_table(n, info, &tab);
+  if (st) return st;
+  const int bytes = tab.hist_words * 4;
+  int waves = (lds_limit - tab.bytes) / bytes;
+  if (waves > 16) waves = 16;
+  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
+  const size_t lds = (size_t)tab.bytes + (size_t)waves * bytes;
+  const int per_cu = (2 * lds <= (size_t)lds_limit) ? 2 : 1;
   long long blocks = ((long long)count + waves - 1) / waves;
   const long long max_blocks = (long long)compute_units * per_cu;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
-  const dim3 grid((unsigned)blocks), block(waves * kWave);
-  if (layout == kPacked16)
-    hipLaunchKernelGGL((score_onesided_kernel<CH, kPacked16>), grid, block, 0, stream, p, q, n, (long long)count,
-                       scores, badness, tab, info, status);
-  else if (layout == kWide32)
-    hipLaunchKernelGGL((score_onesided_kernel<CH, kWide32>), grid, block, lds, stream, p, q, n, (long long)count,
-                       scores, badness, tab, info, status);
-  else if (layout == kNibble16)
-    hipLaunchKernelGGL((score_onesided_kernel<CH, kNibble16>), grid, block, lds, stream, p, q, n, (long long)count,
-                       scores, badness, tab, info, status);
-  else
-    hipLaunchKernelGGL((score_onesided_kernel<CH, kHalf16>), grid, block, lds, stream, p, q, n, (long long)count,
-                       scores, badness, tab, info, status);
+  hipLaunchKernelGGL((score_onesided_kernel<CH>), dim3((unsigned)blocks), dim3(waves * kWave), lds, stream, p, q, n,
+                     (long long)count, scores, badness, tab, info, status);
   return hip_status(hipGetLastError(), "score_onesided_kernel launch");
 }
 
@@ -911,9 +772,9 @@ int score_population_impl(const int32_t* p, const int32_t* q, int n, int64_t cou
     switch (ch) {
       // one chunk: the two-sided histogram needs fewer LDS steps
       case 1: st = launch_slope<1>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      case 2: st = launch_onesided<2>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      case 3: st = launch_onesided<3>(p, q, n, count, scores, badness, table, info, status, stream); break;
-      default: st = launch_onesided<4>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 2: st = launch_onesided<2>(p, q, n, count, scores, badness, info, status, stream); break;
+      case 3: st = launch_onesided<3>(p, q, n, count, scores, badness, info, status, stream); break;
+      default: st = launch_onesided<4>(p, q, n, count, scores, badness, info, status, stream); break;
     }
   } else {
     return set_error(N3_EINVAL, "unknown scoring method %d", method);

@@ -5,17 +5,13 @@
 // slots l, l+64, l+128, ... in registers as keys row*W + column, and a per-n
 // LDS table maps the key difference of two slots to the id of their
 // primitive undirected direction.
-//  - score_onesided_kernel (33 <= n <= 100): anchor a builds a histogram of
-//    the directions to the slots b < a, each b reads back its count, and
-//    every collinear triple is counted once, at its last slot (see the
-//    kernel).  C(m, 2) lookups per state.
-//  - score_slope_kernel (n <= 32, one chunk): anchor a histograms all other
-//    slots; each increment returns the slot's rank among the earlier slots
-//    of its direction, whose sum over the lanes is sum_d C(cnt_d, 2), the
-//    anchor's badness.  m(m-1) lookups per state, fewer LDS steps when the
-//    state fits one chunk.
-// Either way the result is the reference's per-point badness
-// (accelerated.py:34-36) and E = sum_b badness_b / 3 (accelerated.py:33).
+//  - score_onesided_kernel (n <= 100): anchor a builds a histogram of the
+//    directions to the slots b < a, each b reads back its count, and every
+//    collinear triple is counted once, at its last slot (see the kernel).
+//    C(m, 2) lookups per state; slots that are alone on their line through
+//    the anchor skip the histogram.
+// The result is the reference's per-point badness (accelerated.py:34-36)
+// and E = sum_b badness_b / 3 (accelerated.py:33).
 //
 // States with coincident points (P[r] == Q[r]) have no direction for the
 // pair; those states take the determinant path, which counts them exactly
@@ -178,143 +174,6 @@ __global__ void __launch_bounds__(256) score_determinant_kernel(const int32_t* _
     }
     int bad[CH];
     determinant_state<CH>(cols, credit, m, bad);
-    int sum = 0;
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int k = lane + kWave * c;
-      if (k < m) {
-        sum += bad[c];
-        if (badness) badness[s * m + k] = bad[c];
-      }
-    }
-    sum = wave_sum(sum);
-    if (lane == 0) scores[s] = sum / 3;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// direction-histogram path (one wavefront per state)
-// ---------------------------------------------------------------------------
-
-template <int CH>
-__global__ void __launch_bounds__(1024) score_slope_kernel(const int32_t* __restrict__ p, const int32_t* __restrict__ q,
-                                                          int n, long long count, int32_t* __restrict__ scores,
-                                                          int32_t* __restrict__ badness,
-                                                          const uint16_t* __restrict__ table_g, DirTableInfo info,
-                                                          int* __restrict__ status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int waves = blockDim.x / kWave;
-  const int wave = threadIdx.x / kWave, lane = lane_id();
-  uint16_t* table = reinterpret_cast<uint16_t*>(smem);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
-  stage_table(table_g, table, info.entries);
-  for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
-  __syncthreads();
-
-  const int m = 2 * n;
-  const int width = info.width;
-  for (long long s = (long long)blockIdx.x * waves + wave; s < count; s += (long long)gridDim.x * waves) {
-    int key[CH];
-    int outside = 0, coincide = 0;
-#pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      const int k = lane + kWave * c;
-      key[c] = 0;
-      if (k < m) {
-        const int col = (k & 1) ? q[s * n + (k >> 1)] : p[s * n + (k >> 1)];
-        outside |= (col < 0 || col >= n);
-        key[c] = (k >> 1) * width + col;
-      }
-      // slots 2r and 2r+1 sit in neighbouring lanes of the same chunk
-      const int partner = __shfl_xor(key[c], 1, kWave);
-      coincide |= (k < m) && (partner == key[c]);
-    }
-    if (__any(outside)) {
-      if (lane == 0) {
-        scores[s] = -1;
-        atomicOr(status, 1);
-      }
-      continue;
-    }
-    if (__any(coincide)) {
-      // coincident points: exact determinant count, with the histogram area
-      // as scratch for the state's columns (it is cleared again below)
-      int32_t* cols = reinterpret_cast<int32_t*>(hist);
-      int32_t* credit = cols + m;
-#pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int k = lane + kWave * c;
-        if (k < m) cols[k] = key[c] - (k >> 1) * width;
-      }
-      wave_sync();
-      int bad[CH];
-      determinant_state<CH>(cols, credit, m, bad);
-      int sum = 0;
-#pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int k = lane + kWave * c;
-        if (k < m) {
-          sum += bad[c];
-          if (badness) badness[s * m + k] = bad[c];
-        }
-      }
-      sum = wave_sum(sum);
-      if (lane == 0) scores[s] = sum / 3;
-      for (int i = lane; i < 2 * m; i += kWave) hist[i] = 0;
-      wave_sync();
-      continue;
-    }
-
-    // lanes past the last slot carry a valid key (slot 0's); they sit out
-    // the LDS work of the last chunk under the exec mask
-    const int key0 = __builtin_amdgcn_readfirstlane(key[0]);
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-      if (lane + kWave * c >= m) key[c] = key0;
-    // badness of anchor a = sum over its direction histogram of C(cnt, 2) =
-    // sum of the values the lanes' increments return (each point's rank
-    // among the earlier points of its direction); the anchor's own lane hits
-    // the zero-vector bin alone and adds 0
-    // One anchor: histogram build with returned ranks, then clear.  Only the
-    // last chunk can hold lanes past the last slot (m > 64 (CH-1)).
-    auto anchor_part = [&](int anchor_key) -> uint32_t {
-      const int base = info.offset - anchor_key;
-      constexpr int L = CH - 1;  // the only chunk that can hold idle lanes
-      const bool tail = lane + kWave * L < m;
-      uint32_t id[CH], old[CH];
-#pragma unroll
-      for (int c = 0; c < L; ++c) id[c] = table[key[c] + base];
-      id[L] = 0;
-      if (tail) id[L] = table[key[L] + base];
-#pragma unroll
-      for (int c = 0; c < L; ++c) old[c] = atomicAdd(&hist[id[c] >> 2], byte_one(id[c]));
-      old[L] = 0;
-      if (tail) old[L] = atomicAdd(&hist[id[L] >> 2], byte_one(id[L]));
-      wave_sync();
-#pragma unroll
-      for (int c = 0; c < L; ++c) hist[id[c] >> 2] = 0u;
-      if (tail) hist[id[L] >> 2] = 0u;
-      wave_sync();
-      uint32_t part = 0;
-#pragma unroll
-      for (int c = 0; c < CH; ++c) part += __builtin_amdgcn_ubfe(old[c], byte_shift(id[c]), 8);
-      return part;
-    };
-    int bad[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c) bad[c] = 0;
-#pragma unroll
-    for (int ca = 0; ca < CH; ++ca) {
-      const int limit = min(kWave, m - kWave * ca);
-      for (int l = 0; l < limit; l += 2) {
-        // two anchors per reduction: a lane's part is at most 3 x 148 and an
-        // anchor's total at most C(199, 2) < 2^16, so they share a word
-        const uint32_t p0 = anchor_part(__builtin_amdgcn_readlane(key[ca], l));
-        const uint32_t p1 = (l + 1 < limit) ? anchor_part(__builtin_amdgcn_readlane(key[ca], l + 1)) : 0u;
-        const uint32_t total = wave_sum_dpp(p0 | (p1 << 16));
-        bad[ca] = (lane == l) ? (int)(total & 0xFFFFu) : ((lane == l + 1) ? (int)(total >> 16) : bad[ca]);
-      }
-    }
     int sum = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
@@ -686,28 +545,6 @@ static int launch_onesided(const int32_t* p, const int32_t* q, int n, int64_t co
 }
 
 template <int CH>
-static int launch_slope(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores, int32_t* badness,
-                        const uint16_t* table, const DirTableInfo& info, int* status, hipStream_t stream) {
-  int lds_limit = 0, compute_units = 0;
-  int st = device_limits(&lds_limit, &compute_units);
-  if (st) return st;
-  const int table_bytes = table_lds_bytes(info);
-  const int hist_bytes = info.hist_words * 4;
-  int waves = (lds_limit - table_bytes) / hist_bytes;
-  if (waves > 16) waves = 16;
-  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
-  const size_t lds = (size_t)table_bytes + (size_t)waves * hist_bytes;
-  // the LDS budget admits one block per CU; the blocks stride over the states
-  long long blocks = ((long long)count + waves - 1) / waves;
-  const long long max_blocks = (long long)compute_units;
-  if (blocks > max_blocks) blocks = max_blocks;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(score_slope_kernel<CH>, dim3((unsigned)blocks), dim3(waves * kWave), lds, stream, p, q, n,
-                     (long long)count, scores, badness, table, info, status);
-  return hip_status(hipGetLastError(), "score_slope_kernel launch");
-}
-
-template <int CH>
 static int launch_determinant(const int32_t* p, const int32_t* q, int n, int64_t count, int32_t* scores,
                               int32_t* badness, int* status, hipStream_t stream) {
   long long blocks = ((long long)count + 3) / 4;
@@ -770,8 +607,7 @@ int score_population_impl(const int32_t* p, const int32_t* q, int n, int64_t cou
     st = device_dirtable(n, &table, &info);
     if (st) return st;
     switch (ch) {
-      // one chunk: the two-sided histogram needs fewer LDS steps
-      case 1: st = launch_slope<1>(p, q, n, count, scores, badness, table, info, status, stream); break;
+      case 1: st = launch_onesided<1>(p, q, n, count, scores, badness, info, status, stream); break;
       case 2: st = launch_onesided<2>(p, q, n, count, scores, badness, info, status, stream); break;
       case 3: st = launch_onesided<3>(p, q, n, count, scores, badness, info, status, stream); break;
       default: st = launch_onesided<4>(p, q, n, count, scores, badness, info, status, stream); break;

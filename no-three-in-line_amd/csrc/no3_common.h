@@ -10,7 +10,11 @@
 // dir_table[key_b - key_a + (n-1) * W + (n-1)] is the id of the primitive
 // undirected direction of (dr, dc) (verifier_b.py:8-19: divide by the gcd,
 // flip the sign so that dr > 0 or dr == 0 < dc).  Ids are dense in
-// [0, D(n)); the zero vector (coincident points) has id D(n).  Two points b,
+// [0, D(n)); the zero vector (coincident points) has id D(n).  The first
+// ids go to the directions (p, q), p > 0, with max(p, |q|) <= (n - 1) / 2:
+// a line of any other direction meets at most two cells of the grid (a row
+// holds only its own two slots), so a point seen in such a direction from
+// an anchor is alone on its line, and histograms may leave it out.  Two points b,
 // c are collinear with a iff their ids seen from a are equal, so the number
 // of collinear triples through a is sum_d C(cnt_d, 2) over the histogram of
 // ids seen from a (the counting form of verifier_b.find_collinear_triple).
@@ -33,6 +37,10 @@ constexpr uint32_t kTagMove = 0x4D4F5645u;  // one tempering proposal
 constexpr uint32_t kTagBias = 0x42494153u;  // badness-biased slot draw
 constexpr uint32_t kTagExch = 0x45584348u;  // replica exchange
 
+// Trash words at the end of a histogram of the counted ids: one per lane
+// and chunk of 64 slots (n <= 100: four chunks).
+constexpr int kCountedTrashWords = 4 * kWave;
+
 struct DirTableInfo {
   int n;
   int width;        // W = 2n - 1
@@ -43,6 +51,10 @@ struct DirTableInfo {
                     // byte counters, then 64 spare "trash" words, one per
                     // lane, that masked lanes update instead of branching
                     // around the update (one word per lane: no conflicts)
+  int counted;      // ids [0, counted): the directions whose lines can meet
+                    // three cells of the grid (see build_dirtable)
+  int counted_words;  // 32-bit words of a histogram of those ids only and
+                      // kCountedTrashWords trash words, a multiple of 16
 };
 
 __host__ __device__ inline int trash_word(const DirTableInfo& info, int lane) {

@@ -2,6 +2,7 @@
 // per-(device, n) direction tables kept resident in HBM.
 #include "runtime.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cstddef>
 #include <cstdarg>
@@ -30,18 +31,23 @@ int hip_status(hipError_t e, const char* what) {
 
 // verifier_b.py:8-19 without the gcd per query: every primitive undirected
 // direction with |dr|, |dc| <= n-1 gets a dense id, in row-major order of
-// (dr, dc) with dr >= 0.
+// (dr, dc) with dr >= 0, first those whose lines can meet three cells of the
+// grid (dr > 0 and 2 max(dr, |dc|) <= n - 1), then the others.
 int build_dirtable(int n, std::vector<uint16_t>& table, DirTableInfo& info) {
   if (n < 2 || n > kMaxNTable) return set_error(N3_EINVAL, "direction table needs 2 <= n <= %d (got %d)", kMaxNTable, n);
   const int w = 2 * n - 1, span = n - 1;
   std::vector<int> id((size_t)w * w, -1);
-  int next = 0;
-  for (int dr = 0; dr <= span; ++dr) {
-    for (int dc = -span; dc <= span; ++dc) {
-      if (dr == 0 && dc <= 0) continue;
-      if (std::gcd(dr, dc < 0 ? -dc : dc) != 1) continue;
-      id[(size_t)(dr + span) * w + (dc + span)] = next++;
+  int next = 0, counted = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int dr = 0; dr <= span; ++dr) {
+      for (int dc = -span; dc <= span; ++dc) {
+        if (dr == 0 && dc <= 0) continue;
+        if (std::gcd(dr, dc < 0 ? -dc : dc) != 1) continue;
+        const bool three = dr > 0 && 2 * std::max(dr, dc < 0 ? -dc : dc) <= n - 1;
+        if (three == (pass == 0)) id[(size_t)(dr + span) * w + (dc + span)] = next++;
+      }
     }
+    if (pass == 0) counted = next;
   }
   const int zero_id = next;
   table.assign((size_t)w * w, 0);
@@ -74,6 +80,8 @@ int build_dirtable(int n, std::vector<uint16_t>& table, DirTableInfo& info) {
   if (words < 4 * n) words = 4 * n;
   words += kWave;
   info.hist_words = (words + 15) / 16 * 16;
+  info.counted = counted;
+  info.counted_words = ((counted + 3) / 4 + kCountedTrashWords + 15) / 16 * 16;
   return N3_OK;
 }
 

@@ -479,27 +479,22 @@ static int onesided_table(int n, const DirTableInfo& info, OneSidedTable* out) {
   st = build_dirtable(n, ids, built);
   if (st) return st;
   const int w = info.width, span = n - 1;
-  // Counters only for the directions whose lines can meet three cells of
-  // the grid, renumbered in id order.  The zero vector and every other
-  // direction share the word after them, which no lane increments.
-  const uint32_t zero_id = (uint32_t)info.directions;
-  std::vector<int> counter_id(zero_id + 1, -1);
-  for (int dr = 0; dr <= span; ++dr)
-    for (int dc = 0; dc < w; ++dc)
-      if (lone_row_bound(n, dr - span, dc - span) != 255u) counter_id[ids[(size_t)dr * w + dc]] = 0;
-  uint32_t counters = 0;
-  for (uint32_t id = 0; id < zero_id; ++id)
-    if (counter_id[id] == 0) counter_id[id] = (int)counters++;
-  if ((int)counters != onesided_counters(n))
-    return set_error(N3_EINVAL, "one-sided table: %u counters at n=%d, expected %d", counters, n, onesided_counters(n));
-  const uint32_t shared_id = (counters + 3u) & ~3u;
+  // Counters for the ids [0, counted): the directions whose lines can meet
+  // three cells of the grid.  The zero vector and every other direction
+  // share the word after them, which no lane increments.
+  if (built.counted != onesided_counters(n))
+    return set_error(N3_EINVAL, "one-sided table: %d counted directions at n=%d, expected %d", built.counted, n,
+                     onesided_counters(n));
+  const uint32_t counted = (uint32_t)built.counted, shared_id = (counted + 3u) & ~3u;
   const int stride = onesided_stride(n);
   std::vector<uint32_t> host(((size_t)stride * w + 3) / 4 * 4, 0u);  // padded to 16 bytes
   for (int dr = 0; dr <= span; ++dr)  // rows dr - span = -(n-1) .. 0
     for (int dc = 0; dc < w; ++dc) {
       const uint32_t need = lone_row_bound(n, dr - span, dc - span);
       const uint32_t id = ids[(size_t)dr * w + dc];
-      const uint32_t cid = (need == 255u || id == zero_id) ? shared_id : (uint32_t)counter_id[id];
+      if ((need == 255u) != (id >= counted && !(dr == span && dc == span)))
+        return set_error(N3_EINVAL, "one-sided table: lone bound and counted ids disagree at n=%d", n);
+      const uint32_t cid = id < counted ? id : shared_id;
       host[(size_t)dr + (size_t)stride * dc] = ((cid & ~3u) << 16) | (need << 8) | ((cid & 3u) << 3);
     }
   OneSidedTable tab{nullptr, (int)(host.size() * 4), stride, onesided_hist_words(n)};

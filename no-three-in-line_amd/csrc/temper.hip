@@ -79,8 +79,11 @@ __device__ inline double umbrella_energy(int delta, int dd, int removed, float s
 // One replica slot's `a.steps` proposals; `table` and `hist` are this
 // wavefront's LDS direction table and histogram.  kPlain: the caller
 // guarantees no proposal log and no badness bias, and the instantiation
-// carries neither (fewer live scalar values in the loop).
-template <int CH, bool kUmbrella = false, bool kPlain = false>
+// carries neither (fewer live scalar values in the loop).  kCounted (plain
+// only): the histogram holds only the counted directions (DirTableInfo) and
+// a trash word per lane and chunk, a quarter of the LDS, so that more waves
+// fit beside the larger tables.
+template <int CH, bool kUmbrella = false, bool kPlain = false, bool kCounted = false>
 __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t* hist, const DirTableInfo& info,
                              int slot_in, uint32_t sweep) {
   const int lane = lane_id();
@@ -121,6 +124,8 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       if (lane + kWave * c >= m) key[c] = key0;
   }
   const int trash = trash_word(info, lane);
+  // kCounted: trash word of chunk c at counted_trash + 64 c
+  const int counted_trash = info.counted_words - kCountedTrashWords + lane;
   const bool tail_in = lane + kWave * (CH - 1) < m;
   int score = a.scores[slot];
   int removed = a.removed[slot];
@@ -243,20 +248,33 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       idv2[c] = table[moved - key_v2 + offset];
     }
     uint32_t word_u2[CH], word_v2[CH], old_u2[CH], old_v2[CH];
+    if constexpr (kCounted) {
+      // a lane seen in a direction left out of the histogram (the moved
+      // slot's zero vector among them) and the lanes past the last slot are
+      // alone on their line: they count in their trash word (rank 0)
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
-      old_u2[c] = atomicAdd(&hist[word_u2[c]], byte_one((uint32_t)idu2[c]));
+      for (int c = 0; c < CH; ++c) {
+        const int live = (c < CH - 1 || tail_in) ? 1 : 0;
+        word_u2[c] = (live & (int)((uint32_t)idu2[c] < (uint32_t)info.counted)) ? ((uint32_t)idu2[c] >> 2)
+                                                                                 : (uint32_t)(counted_trash + kWave * c);
+        word_v2[c] = (live & (int)((uint32_t)idv2[c] < (uint32_t)info.counted)) ? ((uint32_t)idv2[c] >> 2)
+                                                                                 : (uint32_t)(counted_trash + kWave * c);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        word_u2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idu2[c] >> 2) : (uint32_t)trash;
+        word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
+      }
     }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) old_u2[c] = atomicAdd(&hist[word_u2[c]], byte_one((uint32_t)idu2[c]));
     wave_sync();
 #pragma unroll
     for (int c = 0; c < CH; ++c) hist[word_u2[c]] = 0u;
     wave_sync();
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      word_v2[c] = (c < CH - 1 || tail_in) ? ((uint32_t)idv2[c] >> 2) : (uint32_t)trash;
-      old_v2[c] = atomicAdd(&hist[word_v2[c]], byte_one((uint32_t)idv2[c]));
-    }
+    for (int c = 0; c < CH; ++c) old_v2[c] = atomicAdd(&hist[word_v2[c]], byte_one((uint32_t)idv2[c]));
     wave_sync();
 #pragma unroll
     for (int c = 0; c < CH; ++c) hist[word_v2[c]] = 0u;
@@ -318,16 +336,31 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     bool include[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
-    direction_counts_known<CH>(hist, trash, idu2, include, cnt_u2);
+    if constexpr (kCounted) {
+      direction_counts_counted<CH>(hist, info.counted, counted_trash, idu2, include, cnt_u2);
 #pragma unroll
-    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
-    direction_counts_known<CH>(hist, trash, idv2, include, cnt_v2);
+      for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+      direction_counts_counted<CH>(hist, info.counted, counted_trash, idv2, include, cnt_v2);
 #pragma unroll
-    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
-    direction_counts_known<CH>(hist, trash, idu, include, cnt_u);
+      for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
+      direction_counts_counted<CH>(hist, info.counted, counted_trash, idu, include, cnt_u);
 #pragma unroll
-    for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
-    direction_counts<CH>(table, hist, trash, offset, key_v, key, include, cnt_v, id);
+      for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) id[c] = table[key[c] - key_v + offset];
+      direction_counts_counted<CH>(hist, info.counted, counted_trash, id, include, cnt_v);
+    } else {
+      direction_counts_known<CH>(hist, trash, idu2, include, cnt_u2);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+      direction_counts_known<CH>(hist, trash, idv2, include, cnt_v2);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != u;
+      direction_counts_known<CH>(hist, trash, idu, include, cnt_u);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) include[c] = (lane + kWave * c < m) && (lane + kWave * c) != v;
+      direction_counts<CH>(table, hist, trash, offset, key_v, key, include, cnt_v, id);
+    }
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int k = lane + kWave * c;
@@ -404,6 +437,13 @@ struct BatchGroup {
   int pad;
 };
 
+// Histogram words per wavefront of the batched walks: plain runs count only
+// the counted directions (walk_replica's kCounted), so that the larger sizes
+// fit more waves per block.
+__host__ __device__ inline int batch_hist_words(const DirTableInfo& info, bool plain) {
+  return plain ? info.counted_words : info.hist_words;
+}
+
 // kPlain: no group has a proposal log or a badness bias (see walk_replica).
 template <bool kPlain>
 __global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __restrict__ groups, int group_count) {
@@ -414,19 +454,20 @@ __global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __
   const DirTableInfo info = G.info;
   const int wave = threadIdx.x / kWave, lane = lane_id();
   uint16_t* table = reinterpret_cast<uint16_t*>(smem);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+  const int hist_words = batch_hist_words(info, kPlain);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * hist_words;
   stage_table(G.table, table, info.entries);
   if (wave < G.waves)
-    for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+    for (int i = lane; i < hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
   if (wave >= G.waves) return;
   const int slot = walk_slot((int)blockIdx.x - G.block_begin, wave, G.waves, G.walk.ladder);
   if (slot >= G.walk.replicas) return;
   switch (G.ch) {
-    case 1: walk_replica<1, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    case 2: walk_replica<2, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    case 3: walk_replica<3, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    default: walk_replica<4, false, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 1: walk_replica<1, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 2: walk_replica<2, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 3: walk_replica<3, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    default: walk_replica<4, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
   }
 }
 
@@ -603,11 +644,12 @@ __global__ void __launch_bounds__(1024) walk_persistent_kernel(BatchTables* __re
       const BatchGroup& G = t->walk[g];
       const DirTableInfo info = G.info;
       uint16_t* table = reinterpret_cast<uint16_t*>(smem);
-      uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * info.hist_words;
+      const int hist_words = batch_hist_words(info, kPlain);
+      uint32_t* hist = reinterpret_cast<uint32_t*>(smem + table_lds_bytes(info)) + wave * hist_words;
       if (g != staged) {
         stage_table(G.table, table, info.entries);
         if (wave < G.waves)
-          for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
+          for (int i = lane; i < hist_words; i += kWave) hist[i] = 0;
         staged = g;
         __syncthreads();
       }
@@ -618,10 +660,10 @@ __global__ void __launch_bounds__(1024) walk_persistent_kernel(BatchTables* __re
       if (wave < G.waves && slot < G.walk.replicas) {
         const uint32_t sweep = sweep0[g] + (uint32_t)s;
         switch (G.ch) {
-          case 1: walk_replica<1, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
-          case 2: walk_replica<2, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
-          case 3: walk_replica<3, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
-          default: walk_replica<4, false, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+          case 1: walk_replica<1, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+          case 2: walk_replica<2, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+          case 3: walk_replica<3, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+          default: walk_replica<4, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
         }
       }
       __syncthreads();  // the item is done before the next one may restage the table
@@ -877,6 +919,8 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
       order[j] = order[j - 1];
       order[j - 1] = t;
     }
+  for (int i = 0; i < count; ++i)
+    if (groups[i].chains.log || groups[i].bad_bias > 0) plan->plain = false;
   int blocks = 0, max_waves = 1, exchange_waves = 0;
   size_t lds = 0;
   for (int i = 0; i < count; ++i) {
@@ -885,7 +929,7 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
     BatchGroup& bg = plan->host.walk[i];
     st = device_dirtable(c->n, &bg.table, &bg.info);
     if (st) return st;
-    const int table_bytes = table_lds_bytes(bg.info), hist_bytes = bg.info.hist_words * 4;
+    const int table_bytes = table_lds_bytes(bg.info), hist_bytes = batch_hist_words(bg.info, plan->plain) * 4;
     const int fit = (lds_limit - table_bytes) / hist_bytes;
     int waves = fit < 16 ? fit : 16;
     if (c->block_waves > 0) {
@@ -896,7 +940,6 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
     }
     if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", c->n);
     bg.walk = walk_args(c, gr.steps, gr.bad_bias, gr.seed);
-    if (c->log || gr.bad_bias > 0) plan->plain = false;
     bg.walk.ladder = gr.ladder > 1 ? gr.ladder : 0;
     bg.ch = (2 * c->n + kWave - 1) / kWave;
     bg.waves = waves;

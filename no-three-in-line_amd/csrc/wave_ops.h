@@ -131,6 +131,30 @@ __device__ inline void direction_counts_known(uint32_t* hist, int trash, const i
   wave_sync();
 }
 
+// As direction_counts_known for a histogram of the counted ids only
+// ([0, counted), see DirTableInfo) with a trash word per lane and chunk
+// (word trash + 64 c): a lane that is left out, or seen in any other
+// direction (alone on its line), counts in its trash word instead, so the
+// latter read back 1, and every lane executes every LDS step.
+template <int CH>
+__device__ inline void direction_counts_counted(uint32_t* hist, int counted, int trash, const int (&id)[CH],
+                                                const bool (&include)[CH], int (&cnt)[CH]) {
+  uint32_t word[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    word[c] = ((int)include[c] & (int)((uint32_t)id[c] < (uint32_t)counted)) ? ((uint32_t)id[c] >> 2)
+                                                                             : (uint32_t)(trash + kWave * c);
+    atomicAdd(&hist[word[c]], byte_one((uint32_t)id[c]));
+  }
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], byte_shift((uint32_t)id[c]), 8);
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
+  wave_sync();
+}
+
 __device__ inline void stage_table(const uint16_t* __restrict__ table_g, uint16_t* table, int entries) {
   const uint2* src = reinterpret_cast<const uint2*>(table_g);
   uint2* dst = reinterpret_cast<uint2*>(table);

@@ -39,7 +39,7 @@ __device__ inline long long load_value(const void* base, long long index) {
 }
 
 template <int CH, typename BT, typename OT>
-__global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const uint16_t* __restrict__ table_g,
+__global__ void __launch_bounds__(1024) switch_scores_kernel(SwitchArgs a, const uint16_t* __restrict__ table_g,
                                                             DirTableInfo info) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int waves = blockDim.x / kWave;
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(512) switch_scores_kernel(SwitchArgs a, const 
 //   bad[u] + bad[v] + bad[w] - j(uv) - j(uw) - j(vw) + t
 // uses the caller's badness like the reference.
 template <int CH>
-__global__ void __launch_bounds__(256) three_cycle_kernel(const int32_t* __restrict__ p,
+__global__ void __launch_bounds__(1024) three_cycle_kernel(const int32_t* __restrict__ p,
                                                           const int32_t* __restrict__ q, int n,
                                                           const int32_t* __restrict__ candidates, long long count,
                                                           long long current, const long long* __restrict__ badness,
@@ -321,8 +321,11 @@ static int launch_switch(const SwitchArgs& args, hipStream_t stream) {
   st = device_limits(&lds_limit, &compute_units);
   if (st) return st;
   const int table_bytes = table_lds_bytes(info), hist_bytes = info.hist_words * 4;
+  // up to 16 waves per block (n=75: the 44 KB table and 16 histograms of
+  // 7 KB fill one CU's LDS); a single-state call (150 items at n=75) then
+  // stages the table in 10 blocks of 1,024 threads
   int waves = (lds_limit - table_bytes) / hist_bytes;
-  if (waves > 8) waves = 8;
+  if (waves > 16) waves = 16;
   if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", args.n);
   const long long anchors = args.u_fixed >= 0 ? 1 : 2 * args.n;
   const long long items = args.states * anchors * args.splits;
@@ -459,10 +462,16 @@ extern "C" int n3_evaluate_three_cycles(const int32_t* p, const int32_t* q, cons
   hipStream_t s = as_stream(stream);
   st = hip_status(hipMemsetAsync(status, 0, sizeof(int), s), "hipMemsetAsync(status)");
   if (st) return st;
-  const int waves = 4;
+  // up to 16 waves per block beside the table (a CU's LDS holds one block)
+  int lds_limit = 0, compute_units = 0;
+  st = device_limits(&lds_limit, &compute_units);
+  if (st) return st;
+  int waves = (lds_limit - table_lds_bytes(info)) / (info.hist_words * 4);
+  if (waves > 16) waves = 16;
+  if (waves < 1) return set_error(N3_EINVAL, "direction table for n=%d does not fit in LDS", n);
   const size_t lds = (size_t)table_lds_bytes(info) + (size_t)waves * info.hist_words * 4;
   long long blocks = ((long long)count + waves - 1) / waves;
-  if (blocks > 1024) blocks = 1024;
+  if (blocks > 2 * compute_units) blocks = 2 * compute_units;
   const int ch = (2 * n + kWave - 1) / kWave;
   void (*kernel)(const int32_t*, const int32_t*, int, const int32_t*, long long, long long, const long long*,
                  long long*, int*, const uint16_t*, DirTableInfo) =

@@ -163,6 +163,36 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
   // scalar unit is left to the proposal bookkeeping
   uint32_t move_r[4], bias_r[4] = {0u, 0u, 0u, 0u};
   int batch = -kWave;  // first step of the current batch
+  // kPlain: lane j holds the proposal of step batch + j on the current state
+  // (u | v << 8 | cu << 16 | cv << 24), the badness of its two slots
+  // (bad_u | bad_v << 16, each below 2^16) and, as a lane mask, whether it
+  // is valid.  The batch is planned on the VALU in one pass (the row words
+  // and badness gathered with ds_bpermute), again after every acceptance,
+  // and the walk goes from one valid step to the next (s_ff1), with one
+  // readlane per value instead of the scalar unit's per-step draw
+  // decoding, row-word selects and validity tests.
+  uint32_t plan_uv = 0u, plan_bad = 0u;
+  uint64_t plan_valid = 0ull;
+  auto plan = [&]() {
+    const int pu = (int)mulhi32(move_r[0], (uint32_t)m), pv = (int)mulhi32(move_r[1], (uint32_t)m);
+    auto gather = [&](const int(&arr)[CH], int s) -> int {
+      const int index = (s & (kWave - 1)) * 4;
+      int r = __builtin_amdgcn_ds_bpermute(index, arr[0]);
+#pragma unroll
+      for (int c = 1; c < CH; ++c) {
+        const int t = __builtin_amdgcn_ds_bpermute(index, arr[c]);
+        r = ((s >> 6) == c) ? t : r;
+      }
+      return r;
+    };
+    const uint32_t wu = (uint32_t)gather(row_word, pu & ~1), wv = (uint32_t)gather(row_word, pv & ~1);
+    const int pcu = (int)((wu >> (8 * (pu & 1))) & 0xFF), pcu_other = (int)((wu >> (8 * (~pu & 1))) & 0xFF);
+    const int pcv = (int)((wv >> (8 * (pv & 1))) & 0xFF), pcv_other = (int)((wv >> (8 * (~pv & 1))) & 0xFF);
+    plan_valid = (score > 0 ? ~0ull : 0ull) & lanes(batch + lane < a.steps) & lanes((pu >> 1) != (pv >> 1)) &
+                 lanes(pcu != pcv) & lanes(pcu_other != pcv) & lanes(pcv_other != pcu);
+    plan_uv = (uint32_t)pu | (uint32_t)pv << 8 | (uint32_t)pcu << 16 | (uint32_t)pcv << 24;
+    plan_bad = (uint32_t)gather(bad, pu) | (uint32_t)gather(bad, pv) << 16;
+  };
   for (int step = 0; step < a.steps; ++step) {
     if (step - batch >= kWave) {
       batch = step;
@@ -173,13 +203,34 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagMove, k0, k1, move_r);
       if (!kPlain && a.bad_bias > 0)
         philox4x32_10((uint32_t)(step + lane), sweep, (uint32_t)slot, kTagBias, k0, k1, bias_r);
+      if constexpr (kPlain) plan();
+    }
+    if constexpr (kPlain) {
+      // the next valid step of the batch (invalid steps change nothing)
+      const uint64_t rest = plan_valid >> (step - batch);
+      if (rest == 0ull) {
+        step = batch + kWave - 1;
+        continue;
+      }
+      step += (int)__builtin_ctzll(rest);
     }
     const int draw = step - batch;
     uint32_t r[4];
+    int u, v, cu, cv, cu_other = 0, cv_other = 0, planned_bad = 0;
+    uint32_t word_ru = 0u, word_rv = 0u;
+    if constexpr (kPlain) {
+      const uint32_t uv = (uint32_t)__builtin_amdgcn_readlane((int)plan_uv, draw);
+      planned_bad = __builtin_amdgcn_readlane((int)plan_bad, draw);
+      r[2] = (uint32_t)__builtin_amdgcn_readlane((int)move_r[2], draw);
+      u = (int)(uv & 0xFFu);
+      v = (int)((uv >> 8) & 0xFFu);
+      cu = (int)((uv >> 16) & 0xFFu);
+      cv = (int)(uv >> 24);
+    } else {
 #pragma unroll
     for (int i = 0; i < 4; ++i) r[i] = (uint32_t)__builtin_amdgcn_readlane((int)move_r[i], draw);
-    int u = (int)mulhi32(r[0], (uint32_t)m);
-    if (!kPlain && a.bad_bias > 0 && score > 0 && (int)mulhi32(r[3], 100u) < a.bad_bias) {
+    u = (int)mulhi32(r[0], (uint32_t)m);
+    if (a.bad_bias > 0 && score > 0 && (int)mulhi32(r[3], 100u) < a.bad_bias) {
       // tempering.py:51-55: u drawn in proportion to badness
       const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readlane((int)bias_r[0], draw);
       const int target = (int)mulhi32(rb0, (uint32_t)(3 * score));
@@ -194,20 +245,30 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       }
       u = below < m - 1 ? below : m - 1;
     }
-    const int v = (int)mulhi32(r[1], (uint32_t)m);
+    v = (int)mulhi32(r[1], (uint32_t)m);
+    word_ru = (uint32_t)slot_value<CH>(row_word, u & ~1);
+    word_rv = (uint32_t)slot_value<CH>(row_word, v & ~1);
+    cu = (int)((word_ru >> (8 * (u & 1))) & 0xFF);
+    cu_other = (int)((word_ru >> (8 * (~u & 1))) & 0xFF);
+    cv = (int)((word_rv >> (8 * (v & 1))) & 0xFF);
+    cv_other = (int)((word_rv >> (8 * (~v & 1))) & 0xFF);
+    }
     const int ru = u >> 1, rv = v >> 1;
-    const uint32_t word_ru = (uint32_t)slot_value<CH>(row_word, u & ~1);
-    const uint32_t word_rv = (uint32_t)slot_value<CH>(row_word, v & ~1);
-    const int cu = (int)((word_ru >> (8 * (u & 1))) & 0xFF), cu_other = (int)((word_ru >> (8 * (~u & 1))) & 0xFF);
-    const int cv = (int)((word_rv >> (8 * (v & 1))) & 0xFF), cv_other = (int)((word_rv >> (8 * (~v & 1))) & 0xFF);
     const int key_u = ru * width + cu, key_v = rv * width + cv;
-    const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
-    if (!valid) continue;
+    if constexpr (!kPlain) {
+      const bool valid = score > 0 && ru != rv && cu != cv && cu_other != cv && cv_other != cu;
+      if (!valid) continue;
+    }
     valid_moves += 1;
-    const int ref_u0 = (int)((word_ru >> 16) & 0xFF), ref_u1 = (int)(word_ru >> 24);
-    const int ref_v0 = (int)((word_rv >> 16) & 0xFF), ref_v1 = (int)(word_rv >> 24);
-    const int dd = (cu == ref_u0 || cu == ref_u1) + (cv == ref_v0 || cv == ref_v1) - (cv == ref_u0 || cv == ref_u1) -
-                   (cu == ref_v0 || cu == ref_v1);
+    // removed-point change (tempering.py:86-88): the reference columns of
+    // both rows; the plain walk reads them on acceptance only
+    auto removed_change = [&](uint32_t wu, uint32_t wv) {
+      const int ref_u0 = (int)((wu >> 16) & 0xFF), ref_u1 = (int)(wu >> 24);
+      const int ref_v0 = (int)((wv >> 16) & 0xFF), ref_v1 = (int)(wv >> 24);
+      return (cu == ref_u0 || cu == ref_u1) + (cv == ref_v0 || cv == ref_v1) - (cv == ref_u0 || cv == ref_u1) -
+             (cu == ref_v0 || cu == ref_v1);
+    };
+    const int dd = kPlain ? 0 : removed_change(word_ru, word_rv);
 
     bool others[CH];
 #pragma unroll
@@ -295,7 +356,8 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     // bad'(u') in the low half, bad'(v') in the high half (each < 2^16)
     const uint32_t packed = wave_sum_dpp(part);
     const int bad_u2 = (int)(packed & 0xFFFFu), bad_v2 = (int)(packed >> 16);
-    const int bad_u = slot_value<CH>(bad, u), bad_v = slot_value<CH>(bad, v);
+    const int bad_u = kPlain ? (planned_bad & 0xFFFF) : slot_value<CH>(bad, u);
+    const int bad_v = kPlain ? (int)((uint32_t)planned_bad >> 16) : slot_value<CH>(bad, v);
     if (!exact) {
       // delta >= delta - joint(u, v), and the Metropolis words floor(2^32
       // exp(-d / T)) do not grow with d: a proposal that the bound rejects
@@ -329,6 +391,9 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       e[7] = cu | (cv << 16);
     }
     if (!accept) continue;
+    const int moved_dd =
+        kPlain ? removed_change((uint32_t)slot_value<CH>(row_word, u & ~1), (uint32_t)slot_value<CH>(row_word, v & ~1))
+               : dd;
 
     // badness of the unmoved slots: per-slot counts seen from u', v' in the
     // moved state and from u, v now
@@ -380,7 +445,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
       if ((k >> 1) == rv) row_word[c] = (int)(((uint32_t)row_word[c] & ~(0xFFu << (8 * (v & 1)))) | ((uint32_t)cu << (8 * (v & 1))));
     }
     score += delta;
-    removed += dd;
+    removed += moved_dd;
     accepted += 1;
     if (score < best) {
       best = score;
@@ -390,6 +455,8 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
         if (k < m) a.best_columns[base + k] = key[c] - (k >> 1) * width;
       }
     }
+    // the rest of the batch on the new state
+    if constexpr (kPlain) plan();
   }
 #pragma unroll
   for (int c = 0; c < CH; ++c) {

@@ -158,8 +158,8 @@ __global__ void __launch_bounds__(1024) switch_scores_kernel(SwitchArgs a, const
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int k = lane + kWave * c;
-        part += __builtin_amdgcn_ubfe(old_u2[c], byte_shift((uint32_t)idu2[c]), 8) +
-                __builtin_amdgcn_ubfe(old_v2[c], byte_shift((uint32_t)idv2[c]), 8);
+        part += byte_counter(old_u2[c], (uint32_t)idu2[c]) +
+                byte_counter(old_v2[c], (uint32_t)idv2[c]);
         joint_new += __popcll(lanes((k < m) && k != u && k != v) & lanes(idu2[c] == id_u2v2));
       }
       // bad'(u') + bad'(v') (each below 2^16, so the sum fits one word)
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(1024) three_cycle_kernel(const int32_t* __rest
     uint32_t part_uv = 0, part_w = 0;
     int joint_new = 0;
     auto rank_of = [&](uint32_t old, int id, int k, int anchor_slot) -> uint32_t {
-      return (k < m && k != anchor_slot) ? __builtin_amdgcn_ubfe(old, byte_shift((uint32_t)id), 8) : 0u;
+      return (k < m && k != anchor_slot) ? byte_counter(old, (uint32_t)id) : 0u;
     };
 #pragma unroll
     for (int c = 0; c < CH; ++c) {

@@ -244,8 +244,8 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
     int joint_new = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      part += __builtin_amdgcn_ubfe(old_u2[c], byte_shift((uint32_t)idu2[c]), 8) +
-              (__builtin_amdgcn_ubfe(old_v2[c], byte_shift((uint32_t)idv2[c]), 8) << 16);
+      part += byte_counter(old_u2[c], (uint32_t)idu2[c]) +
+              (byte_counter(old_v2[c], (uint32_t)idv2[c]) << 16);
       joint_new += __popcll(lanes(others[c]) & lanes(idu2[c] == id_u2v2));
     }
     const uint32_t packed = wave_sum_dpp(part);
@@ -589,8 +589,8 @@ __device__ void walk_replica_point(const WalkArgs& a, const PointArgs& P, const 
     int joint_new = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      part += __builtin_amdgcn_ubfe(old_u2[c], byte_shift((uint32_t)idu2[c]), 8) +
-              (__builtin_amdgcn_ubfe(old_v2[c], byte_shift((uint32_t)idv2[c]), 8) << 16);
+      part += byte_counter(old_u2[c], (uint32_t)idu2[c]) +
+              (byte_counter(old_v2[c], (uint32_t)idv2[c]) << 16);
       joint_new += __popcll(lanes(others[c]) & lanes(idu2[c] == id_u2v2));
     }
     const uint32_t packed = wave_sum_dpp(part);

@@ -66,6 +66,10 @@ __host__ __device__ inline int trash_word(const DirTableInfo& info, int lane) {
 // The offset is masked so that no shift count reaches 32.
 __host__ __device__ inline uint32_t byte_shift(uint32_t id) { return (id & 3u) << 3; }
 __host__ __device__ inline uint32_t byte_one(uint32_t id) { return 1u << byte_shift(id); }
+// Counter id's byte of a histogram word.  v_bfe_u32 and its intrinsic read
+// the low five bits of the offset (LLVM's instcombine masks constant offsets
+// the same way), so id << 3 needs no mask.
+__device__ inline uint32_t byte_counter(uint32_t word, uint32_t id) { return __builtin_amdgcn_ubfe(word, id << 3, 8); }
 
 __host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
   return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);

@@ -349,8 +349,7 @@ __device__ void walk_replica(const WalkArgs& a, const uint16_t* table, uint32_t*
     for (int c = 0; c < CH; ++c) {
       // trash words (one per lane, last chunk only) and the zero vector of
       // the moved slot return rank 0
-      part += __builtin_amdgcn_ubfe(old_u2[c], byte_shift((uint32_t)idu2[c]), 8) +
-              (__builtin_amdgcn_ubfe(old_v2[c], byte_shift((uint32_t)idv2[c]), 8) << 16);
+      part += byte_counter(old_u2[c], (uint32_t)idu2[c]) + (byte_counter(old_v2[c], (uint32_t)idv2[c]) << 16);
       joint_new += __popcll(lanes(others[c]) & lanes(idu2[c] == id_u2v2));
     }
     // bad'(u') in the low half, bad'(v') in the high half (each < 2^16)

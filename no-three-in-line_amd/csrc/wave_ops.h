@@ -105,7 +105,7 @@ __device__ inline void direction_counts(const uint16_t* table, uint32_t* hist, i
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], byte_shift((uint32_t)id[c]), 8);
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)byte_counter(hist[word[c]], (uint32_t)id[c]);
   wave_sync();
 #pragma unroll
   for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
@@ -124,7 +124,7 @@ __device__ inline void direction_counts_known(uint32_t* hist, int trash, const i
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], byte_shift((uint32_t)id[c]), 8);
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)byte_counter(hist[word[c]], (uint32_t)id[c]);
   wave_sync();
 #pragma unroll
   for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;
@@ -148,7 +148,7 @@ __device__ inline void direction_counts_counted(uint32_t* hist, int counted, int
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < CH; ++c) cnt[c] = (int)__builtin_amdgcn_ubfe(hist[word[c]], byte_shift((uint32_t)id[c]), 8);
+  for (int c = 0; c < CH; ++c) cnt[c] = (int)byte_counter(hist[word[c]], (uint32_t)id[c]);
   wave_sync();
 #pragma unroll
   for (int c = 0; c < CH; ++c) hist[word[c]] = 0u;

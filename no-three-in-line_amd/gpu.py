@@ -2,8 +2,9 @@
 score_population_badness kernel of gpu_device_search.py).
 
 Host-array entry points copy to the device and back like the reference's
-(gpu.py:69-77); device entry points take torch tensors that are already
-resident in HBM and enqueue on the current stream.
+(gpu.py:69-77), the outputs into page-locked buffers; device entry points
+take torch tensors that are already resident in HBM and enqueue on the
+current stream.
 """
 from __future__ import annotations
 
@@ -61,25 +62,41 @@ def check_status(dev: torch.device | None = None) -> None:
     _lib.check(_lib.lib().n3_score_status(_lib.stream_handle(dev)))
 
 
-def score_population_gpu(p_batch, q_batch) -> np.ndarray:
-    """Return exact integer triple counts for a population of P/Q states (gpu.py:69-77)."""
-    p, q = _validate_batches(p_batch, q_batch)
+def _score_host(p: np.ndarray, q: np.ndarray, with_badness: bool):
+    """Scores (and badness) of host batches: the inputs go in with the
+    runtime's pageable copies (copying them in chunks, on side streams
+    behind the scoring, measured slower: the pageable copies do not overlap
+    the kernels); the outputs come back into page-locked host buffers, which
+    the returned arrays view (39.3 MB of badness at n=75, P=65,536: 4x faster
+    than a copy into pageable memory).  Raises like score_population_device
+    for out-of-range columns."""
+    count, n = p.shape
     dev = _lib.device()
     tp = torch.from_numpy(p).to(dev)
     tq = torch.from_numpy(q).to(dev)
-    return score_population_device(tp, tq).cpu().numpy()
+    bad = torch.empty((count, 2 * n), dtype=torch.int32, device=dev) if with_badness else None
+    scores = score_population_device(tp, tq, badness=bad, check=False)
+    host_scores = torch.empty(count, dtype=torch.int32, pin_memory=True)
+    host_scores.copy_(scores, non_blocking=True)
+    host_bad = None
+    if bad is not None:
+        host_bad = torch.empty((count, 2 * n), dtype=torch.int32, pin_memory=True)
+        host_bad.copy_(bad, non_blocking=True)
+    check_status(dev)  # synchronises the stream, copies included
+    return host_scores.numpy(), None if host_bad is None else host_bad.numpy()
+
+
+def score_population_gpu(p_batch, q_batch) -> np.ndarray:
+    """Return exact integer triple counts for a population of P/Q states (gpu.py:69-77)."""
+    p, q = _validate_batches(p_batch, q_batch)
+    return _score_host(p, q, False)[0]
 
 
 def score_population_badness(p_batch, q_batch) -> tuple[np.ndarray, np.ndarray]:
     """Scores and per-slot badness int32[P][2n] of host batches
     (the outputs of gpu_device_search.py:118-161)."""
     p, q = _validate_batches(p_batch, q_batch)
-    dev = _lib.device()
-    tp = torch.from_numpy(p).to(dev)
-    tq = torch.from_numpy(q).to(dev)
-    bad = torch.empty((p.shape[0], 2 * p.shape[1]), dtype=torch.int32, device=dev)
-    scores = score_population_device(tp, tq, badness=bad)
-    return scores.cpu().numpy(), bad.cpu().numpy()
+    return _score_host(p, q, True)
 
 
 def generate_states(seed: int, n: int, count: int, first_index: int = 0,

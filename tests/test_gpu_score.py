@@ -133,6 +133,27 @@ def test_host_entry_point(pkg, oracle, dev):
     assert np.array_equal(s, ref_scores) and np.array_equal(b, ref_bad)
 
 
+def test_host_entry_points_large_population(pkg, oracle, dev):
+    """A larger host batch (odd size): every state against the oracle, an
+    out-of-range column deep in the batch raises, and the arrays of one call
+    (views of page-locked buffers) stay valid after the next calls."""
+    chunk = 8192
+    count = 2 * chunk + 37
+    hp, hq = oracle.generate_states(5, 33, count)
+    ref_scores, ref_bad = oracle.score_population(hp, hq)
+    s1, b1 = pkg.score_population_badness(hp, hq)
+    s2 = pkg.score_population_gpu(hp[::-1].copy(), hq[::-1].copy())
+    assert np.array_equal(s1, ref_scores) and np.array_equal(b1, ref_bad)
+    assert np.array_equal(s2, ref_scores[::-1])
+    empty = pkg.score_population_gpu(hp[:0], hq[:0])
+    assert empty.shape == (0,)
+    bad_q = hq.copy()
+    bad_q[chunk + 5, 3] = 33
+    with pytest.raises(ValueError):
+        pkg.score_population_badness(hp, bad_q)
+    assert np.array_equal(pkg.score_population_gpu(hp, hq), ref_scores)
+
+
 def test_empty_population(pkg, dev):
     p = torch.empty((0, 75), dtype=torch.int32, device=dev)
     out = pkg.score_population_device(p, p.clone())

@@ -195,6 +195,20 @@ def staging(dev: torch.device, words: int) -> Staging:
     return st
 
 
+def to_host(*tensors: torch.Tensor):
+    """Copies device tensors into page-locked host buffers on the current
+    stream, synchronises it, and returns NumPy views of the buffers (a D2H
+    into pageable memory runs at a fraction of the link's rate)."""
+    out = []
+    for t in tensors:
+        host = torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
+        host.copy_(t, non_blocking=True)
+        out.append(host)
+    if tensors:
+        torch.cuda.current_stream(tensors[0].device).synchronize()
+    return tuple(h.numpy() for h in out)
+
+
 def stream_handle(dev: torch.device) -> int:
     return torch.cuda.current_stream(dev).cuda_stream
 

@@ -75,15 +75,10 @@ def _score_host(p: np.ndarray, q: np.ndarray, with_badness: bool):
     tp = torch.from_numpy(p).to(dev)
     tq = torch.from_numpy(q).to(dev)
     bad = torch.empty((count, 2 * n), dtype=torch.int32, device=dev) if with_badness else None
-    scores = score_population_device(tp, tq, badness=bad, check=False)
-    host_scores = torch.empty(count, dtype=torch.int32, pin_memory=True)
-    host_scores.copy_(scores, non_blocking=True)
-    host_bad = None
-    if bad is not None:
-        host_bad = torch.empty((count, 2 * n), dtype=torch.int32, pin_memory=True)
-        host_bad.copy_(bad, non_blocking=True)
-    check_status(dev)  # synchronises the stream, copies included
-    return host_scores.numpy(), None if host_bad is None else host_bad.numpy()
+    scores = score_population_device(tp, tq, badness=bad)
+    if bad is None:
+        return _lib.to_host(scores)[0], None
+    return _lib.to_host(scores, bad)
 
 
 def score_population_gpu(p_batch, q_batch) -> np.ndarray:

@@ -113,4 +113,4 @@ def delta_table(p_batch, q_batch) -> np.ndarray:
     tq = torch.from_numpy(q_host).to(dev)
     bad = torch.empty((p_host.shape[0], 2 * p_host.shape[1]), dtype=torch.int32, device=dev)
     scores = score_population_device(tp, tq, badness=bad)
-    return delta_table_device(tp, tq, scores, bad).cpu().numpy()
+    return _lib.to_host(delta_table_device(tp, tq, scores, bad))[0]

@@ -142,7 +142,7 @@ def mutate_population_device(parent_p: np.ndarray, parent_q: np.ndarray, *, popu
     cp, cq = mutate_population(p, q, bad, quality_count=quality_count, population=population,
                                active_strength=active_strength, max_strength=max_strength, bad_bias=bad_bias,
                                seed=seed, generation=generation)
-    return cp.cpu().numpy(), cq.cpu().numpy()
+    return _lib.to_host(cp, cq)
 
 
 def verify_zero(state: PairPermutations) -> None:

@@ -75,7 +75,7 @@ __device__ void walk_replica_learned(const WalkArgs& a, const LearnedArgs& L, co
   const int lane = lane_id();
   const int slot = __builtin_amdgcn_readfirstlane(slot_in);  // uniform: scalar registers
   const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
-  const uint32_t sweep = a.counters[0];
+  const uint32_t sweep = a.counters[0] + a.sweep_offset;
   const long long base = (long long)slot * m;
 
   int key[CH], bad[CH], ref[CH];
@@ -446,7 +446,7 @@ __device__ void walk_replica_point(const WalkArgs& a, const PointArgs& P, const 
   const int lane = lane_id();
   const int slot = __builtin_amdgcn_readfirstlane(slot_in);
   const int n = a.n, m = 2 * n, width = info.width, offset = info.offset;
-  const uint32_t sweep = a.counters[0];
+  const uint32_t sweep = a.counters[0] + a.sweep_offset;
   const long long base = (long long)slot * m;
 
   int key[CH], bad[CH], ref[CH];

@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a, const uint16_t* 
   for (int i = lane; i < info.hist_words; i += kWave) hist[i] = 0;
   __syncthreads();
   const int slot = walk_slot((int)blockIdx.x, wave, waves, a.ladder);
-  if (slot < a.replicas) walk_replica<CH, kUmbrella, kPlain>(a, table, hist, info, slot, a.counters[0]);
+  if (slot < a.replicas) walk_replica<CH, kUmbrella, kPlain>(a, table, hist, info, slot, a.counters[0] + a.sweep_offset);
 }
 
 // Several populations of different sizes in one launch: block b belongs to
@@ -550,10 +550,7 @@ struct ExchangeArgs {
   const int8_t* sign;
   int k_exchange;
   uint32_t* counters;
-  // with `done` set, the last block to finish also advances counters[0] by
-  // d_sweep and counters[1] by d_round (see exchange_kernel)
-  uint32_t* done;
-  uint32_t d_sweep, d_round;
+  uint32_t round_offset;  // the round number is counters[1] + round_offset
 };
 
 // One wavefront per candidate pair (a, a+1); pairs start at round % 2 inside
@@ -603,28 +600,18 @@ __device__ bool exchange_pair(const ExchangeArgs& x, int wave, uint32_t round) {
   return true;
 }
 
-// The block's swaps are counted with one atomic.  With x.done set, each block
-// then adds 1 to *done, and the block that brings it to gridDim.x -- after
-// every block has read the round -- advances the sweep and round counters
-// and resets *done: one launch per sweep instead of two counter kernels.
+// The block's swaps are counted with one atomic; the round counter is
+// advanced by a counter kernel (once per captured graph of many sweeps).
 __global__ void __launch_bounds__(1024) exchange_kernel(ExchangeArgs x) {
   __shared__ uint32_t swaps;
   if (threadIdx.x == 0) swaps = 0;
   __syncthreads();
-  const bool swapped = exchange_pair(x, (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave), x.counters[1]);
+  const bool swapped =
+      exchange_pair(x, (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave), x.counters[1] + x.round_offset);
   if (swapped && lane_id() == 0) atomicAdd(&swaps, 1u);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (swaps) atomicAdd(reinterpret_cast<unsigned long long*>(x.counters + 2), (unsigned long long)swaps);
-    if (x.done) {
-      __threadfence();
-      if (atomicAdd(x.done, 1u) == gridDim.x - 1) {
-        atomicAdd(&x.counters[0], x.d_sweep);
-        atomicAdd(&x.counters[1], x.d_round);
-        atomicExch(x.done, 0u);
-      }
-    }
-  }
+  if (threadIdx.x == 0 && swaps)
+    atomicAdd(reinterpret_cast<unsigned long long*>(x.counters + 2), (unsigned long long)swaps);
 }
 
 // Device-side description of a batched run (one allocation per plan).
@@ -841,12 +828,13 @@ WalkArgs walk_args(const n3_chains* c, int steps, int bad_bias, uint64_t seed) {
                   c->centers, c->strengths, c->inverse_temperatures};
 }
 
-// The walk; with bump_sweep the sweep counter is advanced by a counter kernel
-// after it (a following fused exchange may advance it instead).
+// The walk of sweep counters[0] + sweep_offset; with bump_sweep the sweep
+// counter is advanced by a counter kernel after it.
 static int enqueue_advance(const n3_chains* c, const WalkPlan& plan, int steps, int bad_bias, uint64_t seed,
-                           hipStream_t stream, bool bump_sweep = true, int ladder = 0) {
+                           hipStream_t stream, bool bump_sweep = true, int ladder = 0, uint32_t sweep_offset = 0) {
   WalkArgs a = walk_args(c, steps, bad_bias, seed);
   a.ladder = ladder;
+  a.sweep_offset = sweep_offset;
   hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream, a, plan.table,
                      plan.info);
   int st = hip_status(hipGetLastError(), "walk_kernel launch");
@@ -855,21 +843,21 @@ static int enqueue_advance(const n3_chains* c, const WalkPlan& plan, int steps, 
   return hip_status(hipGetLastError(), "bump_kernel launch");
 }
 
-int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream, uint32_t* done,
-                     uint32_t d_sweep) {
+int enqueue_exchange(const n3_chains* c, int ladder, uint64_t seed, hipStream_t stream, uint32_t round_offset,
+                     bool advance) {
   const int pairs = (c->replicas / ladder) * (ladder / 2);
   ExchangeArgs x{c->n, c->replicas, ladder, (uint32_t)seed, (uint32_t)(seed >> 32), c->columns, c->badness,
                  c->scores, c->removed_points, c->exchange_thresholds, c->exchange_sign, c->k_exchange, c->counters,
-                 pairs > 0 ? done : nullptr, d_sweep, 1u};
+                 round_offset};
   if (pairs > 0) {
     const int threads = 1024, per_block = threads / kWave;  // few blocks: few counter atomics
     hipLaunchKernelGGL(exchange_kernel, dim3((unsigned)((pairs + per_block - 1) / per_block)), dim3(threads), 0, stream,
                        x);
     int st = hip_status(hipGetLastError(), "exchange_kernel launch");
     if (st) return st;
-    if (x.done) return N3_OK;
   }
-  hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, d_sweep, 1u);
+  if (!advance) return N3_OK;
+  hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, 0u, 1u);
   return hip_status(hipGetLastError(), "bump_kernel launch");
 }
 
@@ -904,7 +892,6 @@ struct GraphEntry {
   GraphKey key;
   hipGraphExec_t exec;
   hipGraph_t graph;
-  uint32_t* done;  // the fused exchange's last-block counter (zero between launches)
 };
 
 static std::mutex g_graph_mutex;
@@ -920,22 +907,27 @@ static void drop_oldest_graph() {
   (void)hipStreamSynchronize(e.key.stream);
   (void)hipGraphExecDestroy(e.exec);
   (void)hipGraphDestroy(e.graph);
-  (void)hipFree(e.done);
   g_graphs.erase(g_graphs.begin());
 }
 
-// A graph of `sweeps` x (walk, exchange): the exchange kernel advances both
-// counters through the entry's `done` word, so a sweep is two launches.
+// A graph of `sweeps` x (walk, exchange) and one counter kernel: the k-th
+// walk and exchange take their sweep and round numbers as the counters plus
+// k, and the counters advance once, by `sweeps`, at the end.
 static int capture_graph(const n3_chains* c, const WalkPlan& plan, int sweeps, int steps, int ladder, int bad_bias,
-                         uint64_t seed, uint32_t* done, hipGraph_t* graph, hipGraphExec_t* exec) {
+                         uint64_t seed, hipGraph_t* graph, hipGraphExec_t* exec) {
   hipStream_t cap = nullptr;
   int st = hip_status(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
   if (st) return st;
   st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (!st) {
     for (int i = 0; i < sweeps && !st; ++i) {
-      st = enqueue_advance(c, plan, steps, bad_bias, seed, cap, ladder <= 1, ladder);
-      if (!st && ladder > 1) st = enqueue_exchange(c, ladder, seed, cap, done, 1u);
+      st = enqueue_advance(c, plan, steps, bad_bias, seed, cap, false, ladder, (uint32_t)i);
+      if (!st && ladder > 1) st = enqueue_exchange(c, ladder, seed, cap, (uint32_t)i, false);
+    }
+    if (!st) {
+      hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, cap, c->counters, (uint32_t)sweeps,
+                         ladder > 1 ? (uint32_t)sweeps : 0u);
+      st = hip_status(hipGetLastError(), "bump_kernel launch");
     }
     hipGraph_t g = nullptr;
     const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
@@ -1290,18 +1282,11 @@ extern "C" int n3_temper_run(const n3_chains* chains, int sweeps, int steps, int
     for (const auto& e : g_graphs)
       if (same_key(e.key, key)) exec = e.exec;
     if (!exec) {
-      uint32_t* done = nullptr;
-      st = hip_status(hipMalloc(&done, sizeof(uint32_t)), "hipMalloc(exchange counter)");
-      if (st) return st;
-      st = hip_status(hipMemset(done, 0, sizeof(uint32_t)), "hipMemset(exchange counter)");
       hipGraph_t graph = nullptr;
-      if (!st) st = capture_graph(chains, plan, per_graph, steps, ladder, bad_bias, seed, done, &graph, &exec);
-      if (st) {
-        (void)hipFree(done);
-        return st;
-      }
+      st = capture_graph(chains, plan, per_graph, steps, ladder, bad_bias, seed, &graph, &exec);
+      if (st) return st;
       if (g_graphs.size() >= kMaxCachedGraphs) drop_oldest_graph();
-      g_graphs.push_back(GraphEntry{key, exec, graph, done});
+      g_graphs.push_back(GraphEntry{key, exec, graph});
     }
   }
   for (int i = 0; i < full; ++i) {
@@ -1322,7 +1307,6 @@ extern "C" void n3_temper_release(void) {
   for (auto& e : g_graphs) {
     (void)hipGraphExecDestroy(e.exec);
     (void)hipGraphDestroy(e.graph);
-    (void)hipFree(e.done);
   }
   g_graphs.clear();
   for (auto& e : g_persistent) {

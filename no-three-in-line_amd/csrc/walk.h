@@ -33,6 +33,7 @@ struct WalkArgs {
   const float* strengths;
   const double* inverse_temperatures;
   int ladder;                     // > 0: each block takes temperatures spread over one ladder (walk_slot)
+  uint32_t sweep_offset;          // the sweep number is counters[0] + sweep_offset (the k-th walk of a graph: k)
 };
 
 }  // namespace no3

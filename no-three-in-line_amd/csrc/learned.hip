@@ -781,14 +781,17 @@ static int learned_plan(const n3_chains* c, LearnedPlan* plan) {
   return N3_OK;
 }
 
+// The walk of sweep counters[0] + sweep_offset; with `advance` a counter
+// kernel advances counters[0] after it.
 static int enqueue_learned(const n3_chains* c, const LearnedPlan& plan, const LearnedArgs& L, int steps, uint64_t seed,
-                           hipStream_t stream, int ladder) {
+                           hipStream_t stream, int ladder, uint32_t sweep_offset = 0, bool advance = true) {
   WalkArgs a = walk_args(c, steps, 0, seed);
   a.ladder = ladder;
+  a.sweep_offset = sweep_offset;
   hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream, a, L, plan.table,
                      plan.info);
   int st = hip_status(hipGetLastError(), "walk_learned_kernel launch");
-  if (st) return st;
+  if (st || !advance) return st;
   hipLaunchKernelGGL(bump_sweep_kernel, dim3(1), dim3(64), 0, stream, c->counters);
   return hip_status(hipGetLastError(), "bump_sweep_kernel launch");
 }
@@ -833,14 +836,17 @@ static int point_plan(const n3_chains* c, PointPlan* plan) {
   return N3_OK;
 }
 
+// The walk of sweep counters[0] + sweep_offset; with `advance` a counter
+// kernel advances counters[0] after it.
 static int enqueue_point(const n3_chains* c, const PointPlan& plan, const PointArgs& P, int steps, uint64_t seed,
-                         hipStream_t stream, int ladder) {
+                         hipStream_t stream, int ladder, uint32_t sweep_offset = 0, bool advance = true) {
   WalkArgs a = walk_args(c, steps, 0, seed);
   a.ladder = ladder;
+  a.sweep_offset = sweep_offset;
   hipLaunchKernelGGL(plan.kernel, dim3(plan.blocks), dim3(plan.waves * kWave), plan.lds, stream, a, P, plan.table,
                      plan.info);
   int st = hip_status(hipGetLastError(), "walk_point_kernel launch");
-  if (st) return st;
+  if (st || !advance) return st;
   hipLaunchKernelGGL(bump_sweep_kernel, dim3(1), dim3(64), 0, stream, c->counters);
   return hip_status(hipGetLastError(), "bump_sweep_kernel launch");
 }
@@ -889,7 +895,10 @@ static int run_graphs(const n3_chains* chains, const void* args, size_t args_siz
     if (st) return st;
     st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     if (!st) {
-      for (int i = 0; i < per_graph && !st; ++i) st = one_sweep(cap);
+      // the k-th sweep of the graph reads its sweep and round numbers as the
+      // counters plus k; one counter kernel advances them at the end
+      for (int i = 0; i < per_graph && !st; ++i) st = one_sweep(cap, (uint32_t)i, false);
+      if (!st) st = enqueue_counters(chains, (uint32_t)per_graph, ladder > 1 ? (uint32_t)per_graph : 0u, cap);
       hipGraph_t g = nullptr;
       const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
       if (!st) st = end;
@@ -918,7 +927,7 @@ static int run_graphs(const n3_chains* chains, const void* args, size_t args_siz
     st = hip_status(hipGraphLaunch(entry->exec, s), "hipGraphLaunch");
     if (st) return st;
   }
-  for (int i = 0; i < rest && !st; ++i) st = one_sweep(s);
+  for (int i = 0; i < rest && !st; ++i) st = one_sweep(s, 0u, true);
   return st;
 }
 
@@ -958,11 +967,12 @@ extern "C" int n3_temper_run_learned(const n3_chains* chains, const uint32_t* we
   st = learned_plan(chains, &plan);
   if (st) return st;
   const LearnedArgs L{weights, temp_index, max_energy, reinterpret_cast<const unsigned long long*>(thr64), k64};
-  return run_graphs(chains, &L, sizeof L, sweeps, steps, ladder, seed, as_stream(stream), [&](hipStream_t s) {
-    int e = enqueue_learned(chains, plan, L, steps, seed, s, ladder);
-    if (!e && ladder > 1) e = enqueue_exchange(chains, ladder, seed, s);
-    return e;
-  });
+  return run_graphs(chains, &L, sizeof L, sweeps, steps, ladder, seed, as_stream(stream),
+                    [&](hipStream_t s, uint32_t k, bool advance) {
+                      int e = enqueue_learned(chains, plan, L, steps, seed, s, ladder, k, advance);
+                      if (!e && ladder > 1) e = enqueue_exchange(chains, ladder, seed, s, k, advance);
+                      return e;
+                    });
 }
 
 extern "C" int n3_temper_run_learned_point(const n3_chains* chains, const uint16_t* params, const float* consts,
@@ -984,11 +994,12 @@ extern "C" int n3_temper_run_learned_point(const n3_chains* chains, const uint16
   st = point_plan(chains, &plan);
   if (st) return st;
   const PointArgs P{consts, params, expw, lt_q, reinterpret_cast<const unsigned long long*>(thr64), k64};
-  return run_graphs(chains, &P, sizeof P, sweeps, steps, ladder, seed, as_stream(stream), [&](hipStream_t s) {
-    int e = enqueue_point(chains, plan, P, steps, seed, s, ladder);
-    if (!e && ladder > 1) e = enqueue_exchange(chains, ladder, seed, s);
-    return e;
-  });
+  return run_graphs(chains, &P, sizeof P, sweeps, steps, ladder, seed, as_stream(stream),
+                    [&](hipStream_t s, uint32_t k, bool advance) {
+                      int e = enqueue_point(chains, plan, P, steps, seed, s, ladder, k, advance);
+                      if (!e && ladder > 1) e = enqueue_exchange(chains, ladder, seed, s, k, advance);
+                      return e;
+                    });
 }
 
 extern "C" int n3_point_weights(const int32_t* columns, const int32_t* badness, const int32_t* scores,

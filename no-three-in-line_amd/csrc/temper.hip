@@ -764,6 +764,11 @@ __global__ void bump_kernel(uint32_t* counters, uint32_t d_sweep, uint32_t d_rou
   }
 }
 
+int enqueue_counters(const n3_chains* c, uint32_t d_sweep, uint32_t d_round, hipStream_t stream) {
+  hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, stream, c->counters, d_sweep, d_round);
+  return hip_status(hipGetLastError(), "bump_kernel launch");
+}
+
 int check_chains(const n3_chains* c) {
   if (!c) return set_error(N3_EINVAL, "null chains");
   if (c->n < 2 || c->n > kMaxNTable) return set_error(N3_EINVAL, "n must be in [2, %d] (got %d)", kMaxNTable, c->n);
@@ -924,11 +929,7 @@ static int capture_graph(const n3_chains* c, const WalkPlan& plan, int sweeps, i
       st = enqueue_advance(c, plan, steps, bad_bias, seed, cap, false, ladder, (uint32_t)i);
       if (!st && ladder > 1) st = enqueue_exchange(c, ladder, seed, cap, (uint32_t)i, false);
     }
-    if (!st) {
-      hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, cap, c->counters, (uint32_t)sweeps,
-                         ladder > 1 ? (uint32_t)sweeps : 0u);
-      st = hip_status(hipGetLastError(), "bump_kernel launch");
-    }
+    if (!st) st = enqueue_counters(c, (uint32_t)sweeps, ladder > 1 ? (uint32_t)sweeps : 0u, cap);
     hipGraph_t g = nullptr;
     const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
     if (!st) st = end;

@@ -511,8 +511,10 @@ __host__ __device__ inline int batch_hist_words(const DirTableInfo& info, bool p
 }
 
 // kPlain: no group has a proposal log or a badness bias (see walk_replica).
+// Every group walks sweep counters[0] + sweep_offset.
 template <bool kPlain>
-__global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __restrict__ groups, int group_count) {
+__global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __restrict__ groups, int group_count,
+                                                            uint32_t sweep_offset) {
   extern __shared__ __align__(16) unsigned char smem[];
   int g = 0;
   while (g + 1 < group_count && (int)blockIdx.x >= groups[g + 1].block_begin) ++g;
@@ -529,11 +531,12 @@ __global__ void __launch_bounds__(1024) walk_batched_kernel(const BatchGroup* __
   if (wave >= G.waves) return;
   const int slot = walk_slot((int)blockIdx.x - G.block_begin, wave, G.waves, G.walk.ladder);
   if (slot >= G.walk.replicas) return;
+  const uint32_t sweep = G.walk.counters[0] + sweep_offset;
   switch (G.ch) {
-    case 1: walk_replica<1, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    case 2: walk_replica<2, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    case 3: walk_replica<3, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
-    default: walk_replica<4, false, kPlain, kPlain>(G.walk, table, hist, info, slot, G.walk.counters[0]); break;
+    case 1: walk_replica<1, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+    case 2: walk_replica<2, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+    case 3: walk_replica<3, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
+    default: walk_replica<4, false, kPlain, kPlain>(G.walk, table, hist, info, slot, sweep); break;
   }
 }
 
@@ -629,7 +632,8 @@ struct BatchTables {
 // As exchange_kernel, for every group of a batched run: swaps are counted per
 // block and group, and the last block to finish advances every group's sweep
 // counter and, for exchanging groups, its round counter.
-__global__ void __launch_bounds__(1024) exchange_batched_kernel(BatchTables* __restrict__ t) {
+__global__ void __launch_bounds__(1024) exchange_batched_kernel(BatchTables* __restrict__ t, uint32_t round_offset,
+                                                                int advance) {
   __shared__ uint32_t swaps[N3_MAX_GROUPS];
   if (threadIdx.x < N3_MAX_GROUPS) swaps[threadIdx.x] = 0;
   __syncthreads();
@@ -637,13 +641,15 @@ __global__ void __launch_bounds__(1024) exchange_batched_kernel(BatchTables* __r
   int g = 0;
   while (g + 1 < t->count && wave >= t->exchange_begin[g + 1]) ++g;
   if (wave < t->exchange_begin[t->count] && t->round_step[g] != 0) {
-    if (exchange_pair(t->exchange[g], wave - t->exchange_begin[g], t->exchange[g].counters[1]) && lane_id() == 0)
+    if (exchange_pair(t->exchange[g], wave - t->exchange_begin[g], t->exchange[g].counters[1] + round_offset) &&
+        lane_id() == 0)
       atomicAdd(&swaps[g], 1u);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < t->count; ++i)
       if (swaps[i]) atomicAdd(reinterpret_cast<unsigned long long*>(t->counters[i] + 2), (unsigned long long)swaps[i]);
+    if (!advance) return;
     __threadfence();
     if (atomicAdd(&t->done, 1u) == gridDim.x - 1) {
       for (int i = 0; i < t->count; ++i) {
@@ -749,11 +755,11 @@ __global__ void __launch_bounds__(1024) walk_persistent_kernel(BatchTables* __re
   }
 }
 
-__global__ void bump_batched_kernel(const BatchTables* __restrict__ t) {
+__global__ void bump_batched_kernel(const BatchTables* __restrict__ t, uint32_t sweeps) {
   const int g = threadIdx.x;
   if (blockIdx.x == 0 && g < t->count) {
-    t->counters[g][0] += 1u;
-    t->counters[g][1] += t->round_step[g];
+    t->counters[g][0] += sweeps;
+    t->counters[g][1] += sweeps * t->round_step[g];
   }
 }
 
@@ -1034,20 +1040,25 @@ static int batch_plan(const n3_group* groups, int count, BatchPlan* plan) {
   return st;
 }
 
-static int enqueue_batched_sweep(const BatchPlan& plan, hipStream_t stream) {
+// One sweep of every group, number counters + offset.  With `advance` the
+// counters move on by one sweep after it (the exchange launch's last block,
+// or a counter kernel); a captured graph passes offsets 0, 1, ... and
+// advances them once at its end instead.
+static int enqueue_batched_sweep(const BatchPlan& plan, hipStream_t stream, uint32_t offset = 0, bool advance = true) {
   // the group array is the first member of the device tables
   static_assert(offsetof(BatchTables, walk) == 0, "walk groups lead the batch tables");
   hipLaunchKernelGGL(plan.plain ? walk_batched_kernel<true> : walk_batched_kernel<false>, dim3(plan.walk_blocks),
                      dim3(plan.walk_threads), plan.walk_lds, stream, reinterpret_cast<const BatchGroup*>(plan.device),
-                     plan.host.count);
+                     plan.host.count, offset);
   int st = hip_status(hipGetLastError(), "walk_batched_kernel launch");
   if (st) return st;
   if (plan.exchange_blocks > 0) {
-    // the exchange launch also advances the counters
-    hipLaunchKernelGGL(exchange_batched_kernel, dim3(plan.exchange_blocks), dim3(1024), 0, stream, plan.device);
+    hipLaunchKernelGGL(exchange_batched_kernel, dim3(plan.exchange_blocks), dim3(1024), 0, stream, plan.device, offset,
+                       advance ? 1 : 0);
     return hip_status(hipGetLastError(), "exchange_batched_kernel launch");
   }
-  hipLaunchKernelGGL(bump_batched_kernel, dim3(1), dim3(64), 0, stream, plan.device);
+  if (!advance) return N3_OK;
+  hipLaunchKernelGGL(bump_batched_kernel, dim3(1), dim3(64), 0, stream, plan.device, 1u);
   return hip_status(hipGetLastError(), "bump_batched_kernel launch");
 }
 
@@ -1116,7 +1127,11 @@ extern "C" int n3_temper_batched_run(const n3_group* groups, int group_count, in
     if (st) return st;
     st = hip_status(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     if (!st) {
-      for (int i = 0; i < per_graph && !st; ++i) st = enqueue_batched_sweep(e.plan, cap);
+      for (int i = 0; i < per_graph && !st; ++i) st = enqueue_batched_sweep(e.plan, cap, (uint32_t)i, false);
+      if (!st) {
+        hipLaunchKernelGGL(bump_batched_kernel, dim3(1), dim3(64), 0, cap, e.plan.device, (uint32_t)per_graph);
+        st = hip_status(hipGetLastError(), "bump_batched_kernel launch");
+      }
       hipGraph_t g = nullptr;
       const int end = hip_status(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
       if (!st) st = end;
